@@ -1,0 +1,586 @@
+/* ORACLE -- test infrastructure only.
+ *
+ * A plain-C, FP64, CPU restatement of Tonto's restricted direct-SCF J/K build for a
+ * Cartesian basis, used by tests/ (as the checker), by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg.  It is never linked into, loaded by, or called from the
+ * product library (tonto_amd/csrc); the product fails loudly if its HIP library is
+ * missing.
+ *
+ * Reference functions restated here (file:line under the Tonto tree):
+ *   make_r_JK_engine ............ foofiles/molecule.fock.foo:1365-1505
+ *   Schwarz_test ................ foofiles/molecule.fock.foo:3497-3536
+ *   make_max_density_elements ... foofiles/molecule.fock.foo:3601-3633
+ *   make_max_abab_integrals ..... foofiles/molecule.fock.foo:3712-3752
+ *   weight_diagonal_blocks ...... foofiles/molecule.fock.foo:4181-4195
+ *   blockwise_symmetric_fold .... foofiles/molecule.fock.foo:4245-4263
+ *   symmetric_reflect ........... foofiles/mat{intrinsic}.foo:4672-4700
+ *   make_precalculated_data ..... foofiles/shell1pair.foo:100-156
+ *   set_ab_new / set_cd_new ..... foofiles/shell1quartet.foo:131-420 (pair skip rule)
+ *   make_esfs_XX (Rys VRR) ...... foofiles/shell1quartet.foo:7420-7609
+ *   transfer_cd / transfer_ab ... foofiles/shell1quartet.foo:8310-11134 (HRR)
+ *   make_r_JK_engine (quartet) .. foofiles/shell1quartet.foo:11386-11528 (K scatter)
+ * Rys roots/weights: either the FP64 Chebyshev table (tonto_amd/csrc/rys_table.inc,
+ * itself pinned against rys_quad.c) or the quad-precision Golub-Welsch of rys_quad.c.
+ *
+ * J is accumulated directly from (ab|cd) instead of through Tonto's J-engine
+ * (reverse_transfer / forward_transfer, molecule.fock.foo:3823-4150); the two are the
+ * same linear map (the HRR is linear in the (e0| source), so this changes rounding
+ * only.
+ *
+ * Layouts: matrices are n_bf x n_bf, row-major (they are symmetric).  Shell pairs use
+ * Tonto's order: pair index ab = a(a+1)/2 + b (0-based, a >= b).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#include "../tonto_amd/csrc/rys_table.inc"
+
+int orq_rys(int n, double X, double *u, double *w);
+
+#define TWOPI5ON2 34.9868366552497256925   /* include/macros.in:446 */
+#define LGMAX 8                             /* component tables up to l = 8 */
+#define NGMAX 165                           /* n_comp_up_to(8) */
+#define MAXPRIM 16
+
+typedef struct {
+    int n_shell, n_atom, n_bf;
+    const int *l, *nprim, *atom, *first_bf, *prim_off;
+    const double *exps, *coefs, *atom_xyz, *atom_Z;
+} osys_t;
+
+typedef struct {
+    double schwarz, jp, kp, pair;   /* ERI_Schwarz_cutoff, ERI_J/K_density_cutoff, pair_cutoff */
+    int rys_quad;                   /* 1: quad-precision Golub-Welsch roots */
+} ocut_t;
+
+/* ---------------- Cartesian component tables (gaussian_data.foo:295-479) ----- */
+static int g_ready = 0;
+static int g_xyz[NGMAX][3], g_l[NGMAX], g_p1[NGMAX][3];
+static int g_index[LGMAX + 2][LGMAX + 2][LGMAX + 2];
+
+static int ncomp(int l) { return (l + 1) * (l + 2) / 2; }
+static int ncomp_up_to(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }
+
+static void comp_tables(void)
+{
+    if (g_ready) return;
+    int i = 0;
+    for (int L = 0; L <= LGMAX; L++) {
+        for (int a = L; a >= (L + 2) / 3; a--) {
+            for (int b = (L - a < a ? L - a : a); b >= (L - a + 1) / 2; b--) {
+                int c = L - a - b, pat[6][3], np;
+                if (a == b && b == c) { np = 1; int t[1][3] = {{a, a, a}}; memcpy(pat, t, sizeof t); }
+                else if (a > b && b == c) { np = 3; int t[3][3] = {{a, b, b}, {b, a, b}, {b, b, a}}; memcpy(pat, t, sizeof t); }
+                else if (a == b && b > c) { np = 3; int t[3][3] = {{a, a, c}, {a, c, a}, {c, a, a}}; memcpy(pat, t, sizeof t); }
+                else { np = 6; int t[6][3] = {{a, b, c}, {a, c, b}, {b, a, c}, {c, a, b}, {b, c, a}, {c, b, a}}; memcpy(pat, t, sizeof t); }
+                for (int k = 0; k < np; k++, i++) {
+                    g_xyz[i][0] = pat[k][0]; g_xyz[i][1] = pat[k][1]; g_xyz[i][2] = pat[k][2];
+                    g_l[i] = L;
+                    g_index[pat[k][0]][pat[k][1]][pat[k][2]] = i;
+                }
+            }
+        }
+    }
+    for (int g = 0; g < NGMAX; g++)
+        for (int t = 0; t < 3; t++) {
+            int x[3] = {g_xyz[g][0], g_xyz[g][1], g_xyz[g][2]};
+            x[t]++;
+            g_p1[g][t] = (g_l[g] < LGMAX) ? g_index[x[0]][x[1]][x[2]] : -1;
+        }
+    g_ready = 1;
+}
+
+/* component (lx,ly,lz) of the i-th member of an l-shell */
+void orc_component(int l, int i, int *xyz)
+{
+    comp_tables();
+    int g = ncomp_up_to(l - 1) + i;
+    xyz[0] = g_xyz[g][0]; xyz[1] = g_xyz[g][1]; xyz[2] = g_xyz[g][2];
+}
+
+/* ---------------- Rys roots/weights --------------------------------------------- */
+void orc_rys_table(int n, double X, double *u, double *w)
+{
+    if (X >= rys_xasym[n]) {
+        double ix = 1.0 / X, isx = sqrt(ix);
+        for (int i = 0; i < n; i++) {
+            u[i] = rys_herm_s2[(n - 1) * RYS_NMAX + i] * ix;
+            w[i] = rys_herm_w[(n - 1) * RYS_NMAX + i] * isx;
+        }
+        return;
+    }
+    int k = (int)(X * (1.0 / RYS_H));
+    double t = (X - k * RYS_H) * (2.0 / RYS_H) - 1.0;
+    const double *c = rys_table_data + rys_table_offset[n] + (size_t)k * RYS_NCHEB * 2 * n;
+    for (int f = 0; f < 2 * n; f++) {
+        double b1 = 0.0, b2 = 0.0;
+        for (int m = RYS_NCHEB - 1; m >= 1; m--) {
+            double b0 = 2.0 * t * b1 - b2 + c[m * 2 * n + f];
+            b2 = b1; b1 = b0;
+        }
+        double v = t * b1 - b2 + c[f];
+        if (f < n) u[f] = v; else w[f - n] = v;
+    }
+}
+
+static void rys(int n, double X, double *u, double *w, int quad)
+{
+    if (quad) orq_rys(n, X, u, w);
+    else orc_rys_table(n, X, u, w);
+}
+
+/* ---------------- primitive-pair data (shell1pair.foo:100-156, shell1quartet.foo:131-275) */
+typedef struct {
+    int n;                 /* surviving primitive pairs */
+    double zeta[MAXPRIM * MAXPRIM], K[MAXPRIM * MAXPRIM];
+    double P[MAXPRIM * MAXPRIM][3], PA[MAXPRIM * MAXPRIM][3];
+    double R[3];           /* big - small centre (HRR) */
+    int big_is_first;      /* Tonto: At = pos_a if la > lb else pos_b */
+} pairdat_t;
+
+static void make_pair(const osys_t *s, int a, int b, int skip, double pair_cut, pairdat_t *pd)
+{
+    const double *A = s->atom_xyz + 3 * s->atom[a], *B = s->atom_xyz + 3 * s->atom[b];
+    int same = s->atom[a] == s->atom[b];
+    double AB[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+    double r2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    pd->big_is_first = s->l[a] > s->l[b];
+    const double *At = pd->big_is_first ? A : B;
+    const double *Sm = pd->big_is_first ? B : A;
+    for (int t = 0; t < 3; t++) pd->R[t] = At[t] - Sm[t];
+    int j = 0;
+    const double *ea = s->exps + s->prim_off[a], *ca = s->coefs + s->prim_off[a];
+    const double *eb = s->exps + s->prim_off[b], *cb = s->coefs + s->prim_off[b];
+    for (int bg = 0; bg < s->nprim[b]; bg++) {
+        double be = eb[bg], b_r2 = be * r2;
+        for (int ag = 0; ag < s->nprim[a]; ag++) {
+            double ae = ea[ag];
+            double sum = ae + be, inv = 1.0 / sum;
+            double cc = cb[bg] * ca[ag] * inv * sqrt(inv);
+            double pref;
+            if (same && !skip) {
+                pref = cc;
+            } else {
+                double e = exp(-ae * b_r2 * inv);
+                if (skip && e < pair_cut) continue;
+                pref = cc * e;
+            }
+            pd->zeta[j] = sum;
+            pd->K[j] = pref;
+            for (int t = 0; t < 3; t++) {
+                pd->P[j][t] = same && !skip ? A[t] : (be * B[t] + ae * A[t]) * inv;
+                pd->PA[j][t] = pd->P[j][t] - At[t];
+            }
+            j++;
+        }
+    }
+    pd->n = j;
+}
+
+/* ---------------- HRR (transfer relations) ------------------------------------ */
+/* in : [lead][comps with l in lbig..lbig+lsm][trail]
+ * out: [lead][ncomp(lbig)][ncomp(lsm)][trail]
+ * (a, b+1_t) = (a+1_t, b) + R_t (a, b), R = big - small centre. */
+static void hrr(int lbig, int lsm, const double *R, const double *in, int lead, int trail,
+                double *out)
+{
+    int base = ncomp_up_to(lbig - 1);
+    int nin = ncomp_up_to(lbig + lsm) - base;
+    if (lsm == 0) {
+        memcpy(out, in, sizeof(double) * lead * nin * trail);
+        return;
+    }
+    /* level k: a in l range [lbig, lbig+lsm-k] (na_k comps), b comps of l = k */
+    double *cur = malloc(sizeof(double) * lead * nin * trail);
+    memcpy(cur, in, sizeof(double) * lead * nin * trail);
+    int na = nin, nb = 1;
+    for (int k = 0; k < lsm; k++) {
+        int na1 = ncomp_up_to(lbig + lsm - k - 1) - base, nb1 = ncomp(k + 1);
+        double *nxt = malloc(sizeof(double) * lead * na1 * nb1 * trail);
+        int gb0 = ncomp_up_to(k - 1), gb1 = ncomp_up_to(k);
+        for (int ib1 = 0; ib1 < nb1; ib1++) {
+            int g1 = gb1 + ib1, t = 0;
+            while (g_xyz[g1][t] == 0) t++;
+            int xb[3] = {g_xyz[g1][0], g_xyz[g1][1], g_xyz[g1][2]};
+            xb[t]--;
+            int ib = g_index[xb[0]][xb[1]][xb[2]] - gb0;
+            for (int L = 0; L < lead; L++)
+                for (int ia = 0; ia < na1; ia++) {
+                    int ga = base + ia, ia1 = g_p1[ga][t] - base;
+                    for (int T = 0; T < trail; T++)
+                        nxt[((L * na1 + ia) * nb1 + ib1) * trail + T] =
+                            cur[((L * na + ia1) * nb + ib) * trail + T] +
+                            R[t] * cur[((L * na + ia) * nb + ib) * trail + T];
+                }
+        }
+        free(cur);
+        cur = nxt; na = na1; nb = nb1;
+    }
+    /* na == ncomp(lbig) */
+    memcpy(out, cur, sizeof(double) * lead * na * nb * trail);
+    free(cur);
+}
+
+/* ---------------- one shell quartet (ab|cd) ------------------------------------- */
+/* out[((ia*nb + ib)*nc + ic)*nd + id]; returns number of primitive quartets */
+static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const ocut_t *cut,
+                    double *out)
+{
+    comp_tables();
+    static __thread pairdat_t pab, pcd;
+    make_pair(s, a, b, skip, cut->pair, &pab);
+    make_pair(s, c, d, skip, cut->pair, &pcd);
+    int la = s->l[a], lb = s->l[b], lc = s->l[c], ld = s->l[d];
+    int lab = la + lb, lcd = lc + ld;
+    int lbig_ab = la > lb ? la : lb, lsm_ab = la > lb ? lb : la;
+    int lbig_cd = lc > ld ? lc : ld, lsm_cd = lc > ld ? ld : lc;
+    int e0 = ncomp_up_to(lbig_ab - 1), ne = ncomp_up_to(lab) - e0;
+    int f0 = ncomp_up_to(lbig_cd - 1), nf = ncomp_up_to(lcd) - f0;
+    int nroot = (lab + lcd) / 2 + 1;
+    double *G = calloc((size_t)ne * nf, sizeof(double));
+    double Ix[9][9], Iy[9][9], Iz[9][9];
+    double u[16], w[16];
+    for (int p = 0; p < pab.n; p++) {
+        double zeta = pab.zeta[p], zinv = 1.0 / zeta;
+        for (int q = 0; q < pcd.n; q++) {
+            double eta = pcd.zeta[q], einv = 1.0 / eta;
+            double rho = zeta * eta / (zeta + eta);
+            double QP[3] = {pcd.P[q][0] - pab.P[p][0], pcd.P[q][1] - pab.P[p][1],
+                            pcd.P[q][2] - pab.P[p][2]};
+            double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
+            rys(nroot, X, u, w, cut->rys_quad);
+            double pref = TWOPI5ON2 * pab.K[p] * pcd.K[q] * sqrt(rho);
+            for (int r = 0; r < nroot; r++) {
+                double t2 = u[r];
+                double rz = t2 * rho * zinv, re = t2 * rho * einv;
+                double b10 = (1.0 - rz) * 0.5 * zinv, b01 = (1.0 - re) * 0.5 * einv;
+                double b00 = rz * 0.5 * einv;
+                double (*I[3])[9] = {Ix, Iy, Iz};
+                for (int t = 0; t < 3; t++) {
+                    double (*M)[9] = I[t];
+                    double c00 = pab.PA[p][t] + rz * QP[t];
+                    double d00 = pcd.PA[q][t] - re * QP[t];
+                    M[0][0] = 1.0;
+                    for (int e = 0; e < lab; e++)
+                        M[e + 1][0] = c00 * M[e][0] + (e ? e * b10 * M[e - 1][0] : 0.0);
+                    for (int f = 0; f < lcd; f++)
+                        for (int e = 0; e <= lab; e++)
+                            M[e][f + 1] = d00 * M[e][f] + (f ? f * b01 * M[e][f - 1] : 0.0) +
+                                          (e ? e * b00 * M[e - 1][f] : 0.0);
+                }
+                double wt = w[r] * pref;
+                for (int ie = 0; ie < ne; ie++) {
+                    const int *xe = g_xyz[e0 + ie];
+                    for (int jf = 0; jf < nf; jf++) {
+                        const int *xf = g_xyz[f0 + jf];
+                        G[ie * nf + jf] += Ix[xe[0]][xf[0]] * Iy[xe[1]][xf[1]] * Iz[xe[2]][xf[2]] * wt;
+                    }
+                }
+            }
+        }
+    }
+    long nprimq = (long)pab.n * pcd.n;
+    /* HRR on cd (trailing index), then ab */
+    int nc = ncomp(lc), nd = ncomp(ld), na_ = ncomp(la), nb_ = ncomp(lb);
+    int nbig_cd = ncomp(lbig_cd), nsm_cd = ncomp(lsm_cd);
+    double *escd = malloc(sizeof(double) * ne * nbig_cd * nsm_cd);
+    hrr(lbig_cd, lsm_cd, pcd.R, G, ne, 1, escd);            /* [e][big][small] */
+    int nbig_ab = ncomp(lbig_ab), nsm_ab = ncomp(lsm_ab);
+    double *abcd = malloc(sizeof(double) * nbig_ab * nsm_ab * nbig_cd * nsm_cd);
+    hrr(lbig_ab, lsm_ab, pab.R, escd, 1, nbig_cd * nsm_cd, abcd); /* [big][small][cdbig][cdsmall] */
+    for (int ia = 0; ia < na_; ia++)
+        for (int ib = 0; ib < nb_; ib++)
+            for (int ic = 0; ic < nc; ic++)
+                for (int id = 0; id < nd; id++) {
+                    int i1 = pab.big_is_first ? ia : ib, i2 = pab.big_is_first ? ib : ia;
+                    int i3 = pcd.big_is_first ? ic : id, i4 = pcd.big_is_first ? id : ic;
+                    out[((ia * nb_ + ib) * nc + ic) * nd + id] =
+                        abcd[((i1 * nsm_ab + i2) * nbig_cd + i3) * nsm_cd + i4];
+                }
+    free(abcd); free(escd); free(G);
+    return nprimq;
+}
+
+/* Public: one Cartesian shell quartet, with or without primitive-pair skipping. */
+long orc_eri_quartet(const osys_t *s, int a, int b, int c, int d, int skip, double pair_cut,
+                     int rys_quad, double *out)
+{
+    ocut_t cut = {0, 0, 0, pair_cut, rys_quad};
+    return quartet(s, a, b, c, d, skip, &cut, out);
+}
+
+/* ---------------- Schwarz integrals (molecule.fock.foo:3712-3752) ---------------- */
+void orc_schwarz(const osys_t *s, int rys_quad, double *max_I)
+{
+    int ns = s->n_shell;
+    ocut_t cut = {0, 0, 0, 1e-6, rys_quad};
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int a = 0; a < ns; a++) {
+        double *buf = malloc(sizeof(double) * 10 * 10 * 10 * 10);
+        for (int b = 0; b <= a; b++) {
+            quartet(s, a, b, a, b, 0, &cut, buf);
+            int na = ncomp(s->l[a]), nb = ncomp(s->l[b]);
+            double m = 0.0;
+            for (int ia = 0; ia < na; ia++)
+                for (int ib = 0; ib < nb; ib++) {
+                    double v = fabs(buf[((ia * nb + ib) * na + ia) * nb + ib]);
+                    if (v > m) m = v;
+                }
+            max_I[a * (a + 1) / 2 + b] = sqrt(m);
+        }
+        free(buf);
+    }
+}
+
+/* ---------------- max density per shell pair (molecule.fock.foo:3601-3633) ------ */
+static void max_density(const osys_t *s, const double *P, double *Pmax)
+{
+    int ns = s->n_shell, n = s->n_bf;
+    for (int a = 0; a < ns; a++)
+        for (int b = 0; b <= a; b++) {
+            double m = 0.0;
+            for (int i = 0; i < ncomp(s->l[a]); i++)
+                for (int j = 0; j < ncomp(s->l[b]); j++) {
+                    double v = fabs(P[(size_t)(s->first_bf[a] + i) * n + s->first_bf[b] + j]);
+                    if (v > m) m = v;
+                }
+            Pmax[a * (a + 1) / 2 + b] = m;
+        }
+}
+
+static inline int pidx(int x, int y) { return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x; }
+
+/* ---------------- the J/K build (molecule.fock.foo:1365-1505) ------------------- */
+/* stats (may be NULL): [0] quartets computed, [1] primitive quartets, [2] J-skips,
+ * [3] K-skips, [4] quartets using skipped primitive-pair lists */
+void orc_make_r_JK(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
+                   double *J, double *K, long *stats)
+{
+    comp_tables();
+    int ns = s->n_shell, n = s->n_bf;
+    long npair = (long)ns * (ns + 1) / 2;
+    double *max_I = malloc(sizeof(double) * npair);
+    if (max_I_in) memcpy(max_I, max_I_in, sizeof(double) * npair);
+    else orc_schwarz(s, cut->rys_quad, max_I);
+    double *Pmax = malloc(sizeof(double) * npair);
+    max_density(s, P, Pmax);
+    double P_max = 0.0, I_max = 0.0;
+    for (long i = 0; i < npair; i++) {
+        if (Pmax[i] > P_max) P_max = Pmax[i];
+        if (max_I[i] > I_max) I_max = max_I[i];
+    }
+    double IP_max = I_max * P_max;
+    memset(J, 0, sizeof(double) * n * n);
+    memset(K, 0, sizeof(double) * n * n);
+    long st[5] = {0, 0, 0, 0, 0};
+    int nthr = omp_get_max_threads();
+    double **Jt = malloc(sizeof(double *) * nthr), **Kt = malloc(sizeof(double *) * nthr);
+    for (int t = 0; t < nthr; t++) {
+        Jt[t] = calloc((size_t)n * n, sizeof(double));
+        Kt[t] = calloc((size_t)n * n, sizeof(double));
+    }
+#pragma omp parallel reduction(+ : st[:5])
+    {
+        int tid = omp_get_thread_num();
+        double *Jl = Jt[tid], *Kl = Kt[tid];
+        double *I = malloc(sizeof(double) * 10 * 10 * 10 * 10);
+#pragma omp for schedule(dynamic, 1)
+        for (long ab = 0; ab < npair; ab++) {
+            if (max_I[ab] * IP_max < cut->schwarz) continue;
+            int a = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
+            while (a * (a + 1) / 2 > ab) a--;
+            while ((a + 1) * (a + 2) / 2 <= ab) a++;
+            int b = (int)(ab - a * (a + 1) / 2);
+            int atom_a = s->atom[a], atom_b = s->atom[b];
+            int ab_same = atom_a == atom_b;
+            for (long cd = 0; cd <= ab; cd++) {
+                if (max_I[ab] * max_I[cd] * P_max < cut->schwarz) continue;
+                int c = (int)((sqrt(8.0 * cd + 1.0) - 1.0) / 2.0);
+                while (c * (c + 1) / 2 > cd) c--;
+                while ((c + 1) * (c + 2) / 2 <= cd) c++;
+                int d = (int)(cd - c * (c + 1) / 2);
+                int atom_c = s->atom[c], atom_d = s->atom[d];
+                int cd_same = atom_c == atom_d;
+                /* Schwarz_test */
+                double JP = Pmax[ab] > Pmax[cd] ? Pmax[ab] : Pmax[cd];
+                double KP = Pmax[pidx(a, c)];
+                if (Pmax[pidx(a, d)] > KP) KP = Pmax[pidx(a, d)];
+                if (Pmax[pidx(b, c)] > KP) KP = Pmax[pidx(b, c)];
+                if (Pmax[pidx(b, d)] > KP) KP = Pmax[pidx(b, d)];
+                int skip_J = (JP < cut->jp) || (JP * max_I[ab] * max_I[cd] < cut->schwarz);
+                int skip_K = (KP < cut->kp) || (KP * max_I[ab] * max_I[cd] < cut->schwarz);
+                if (skip_J && skip_K) continue;
+                double f = (a == b) ? 0.5 : 1.0;
+                if (c == d) f *= 0.5;
+                if (a == c && b == d) f *= 0.5;
+                int any_same = ab_same || cd_same || atom_a == atom_c || atom_a == atom_d ||
+                               atom_b == atom_c || atom_b == atom_d;
+                long npq = quartet(s, a, b, c, d, !any_same, cut, I);
+                st[0]++; st[1] += npq; st[2] += skip_J; st[3] += skip_K; st[4] += !any_same;
+                int na = ncomp(s->l[a]), nb = ncomp(s->l[b]), nc = ncomp(s->l[c]), nd = ncomp(s->l[d]);
+                int fa = s->first_bf[a], fb = s->first_bf[b], fc = s->first_bf[c], fd = s->first_bf[d];
+                for (int ia = 0; ia < na; ia++)
+                    for (int ib = 0; ib < nb; ib++)
+                        for (int ic = 0; ic < nc; ic++)
+                            for (int id = 0; id < nd; id++) {
+                                double v = f * I[((ia * nb + ib) * nc + ic) * nd + id];
+                                int i = fa + ia, j = fb + ib, k = fc + ic, l = fd + id;
+                                if (!skip_J) {
+                                    Jl[(size_t)i * n + j] += v * P[(size_t)k * n + l];
+                                    Jl[(size_t)k * n + l] += v * P[(size_t)i * n + j];
+                                }
+                                if (!skip_K) {
+                                    Kl[(size_t)i * n + k] += v * P[(size_t)l * n + j];
+                                    Kl[(size_t)i * n + l] += v * P[(size_t)k * n + j];
+                                    Kl[(size_t)j * n + k] += v * P[(size_t)l * n + i];
+                                    Kl[(size_t)j * n + l] += v * P[(size_t)k * n + i];
+                                }
+                            }
+            }
+        }
+        free(I);
+    }
+    for (int t = 0; t < nthr; t++) {
+        for (long i = 0; i < (long)n * n; i++) { J[i] += Jt[t][i]; K[i] += Kt[t][i]; }
+        free(Jt[t]); free(Kt[t]);
+    }
+    free(Jt); free(Kt);
+    /* post-processing: weight_diagonal_blocks(J,2); blockwise_symmetric_fold(K); J = 2J;
+     * symmetric_reflect (lower -> upper) */
+    for (int a = 0; a < ns; a++) {
+        int fa = s->first_bf[a], na = ncomp(s->l[a]);
+        for (int i = 0; i < na; i++)
+            for (int j = 0; j < na; j++) J[(size_t)(fa + i) * n + fa + j] *= 2.0;
+    }
+    double *tmp = malloc(sizeof(double) * 100);
+    for (int a = 0; a < ns; a++)
+        for (int b = 0; b <= a; b++) {
+            int fa = s->first_bf[a], na = ncomp(s->l[a]), fb = s->first_bf[b], nb = ncomp(s->l[b]);
+            for (int i = 0; i < na; i++)
+                for (int j = 0; j < nb; j++) tmp[i * nb + j] = K[(size_t)(fb + j) * n + fa + i];
+            for (int i = 0; i < na; i++)
+                for (int j = 0; j < nb; j++) K[(size_t)(fa + i) * n + fb + j] += tmp[i * nb + j];
+        }
+    free(tmp);
+    for (long i = 0; i < (long)n * n; i++) J[i] *= 2.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++) {
+            J[(size_t)j * n + i] = J[(size_t)i * n + j];
+            K[(size_t)j * n + i] = K[(size_t)i * n + j];
+        }
+    if (stats) memcpy(stats, st, sizeof st);
+    free(max_I); free(Pmax);
+}
+
+/* ---------------- one-electron integrals (molecule.ints.foo:34-415) ------------ */
+/* Overlap and kinetic by Obara-Saika 1-D recursions; nuclear attraction by Rys
+ * quadrature with X = p|PC|^2 (the eta -> infinity limit of the ERI recursion). */
+static void os1d(double PA, double PB, double p, int la, int lb, double S[][12])
+{
+    double h = 0.5 / p;
+    S[0][0] = 1.0;
+    for (int i = 0; i <= la; i++)
+        for (int j = 0; j <= lb; j++) {
+            if (i == 0 && j == 0) continue;
+            if (i > 0)
+                S[i][j] = PA * S[i - 1][j] + h * ((i > 1 ? (i - 1) * S[i - 2][j] : 0.0) +
+                                                  (j > 0 ? j * S[i - 1][j - 1] : 0.0));
+            else
+                S[i][j] = PB * S[i][j - 1] + h * ((j > 1 ? (j - 1) * S[i][j - 2] : 0.0));
+        }
+}
+
+void orc_one_electron(const osys_t *s, double *S, double *T, double *V)
+{
+    comp_tables();
+    int ns = s->n_shell, n = s->n_bf;
+    memset(S, 0, sizeof(double) * n * n);
+    memset(T, 0, sizeof(double) * n * n);
+    memset(V, 0, sizeof(double) * n * n);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int a = 0; a < ns; a++)
+        for (int b = 0; b <= a; b++) {
+            int la = s->l[a], lb = s->l[b], na = ncomp(la), nb = ncomp(lb);
+            const double *A = s->atom_xyz + 3 * s->atom[a], *B = s->atom_xyz + 3 * s->atom[b];
+            double sb[10][10], tb[10][10], vb[10][10];
+            memset(sb, 0, sizeof sb); memset(tb, 0, sizeof tb); memset(vb, 0, sizeof vb);
+            double AB2 = 0;
+            for (int t = 0; t < 3; t++) AB2 += (A[t] - B[t]) * (A[t] - B[t]);
+            for (int ia = 0; ia < s->nprim[a]; ia++)
+                for (int ib = 0; ib < s->nprim[b]; ib++) {
+                    double al = s->exps[s->prim_off[a] + ia], be = s->exps[s->prim_off[b] + ib];
+                    double cc = s->coefs[s->prim_off[a] + ia] * s->coefs[s->prim_off[b] + ib];
+                    double p = al + be, mu = al * be / p;
+                    double Pc[3], Sx[3][12][12];
+                    for (int t = 0; t < 3; t++) {
+                        Pc[t] = (al * A[t] + be * B[t]) / p;
+                        os1d(Pc[t] - A[t], Pc[t] - B[t], p, la + 1, lb + 1, Sx[t]);
+                    }
+                    double pre = cc * exp(-mu * AB2) * pow(M_PI / p, 1.5);
+                    for (int i = 0; i < na; i++)
+                        for (int j = 0; j < nb; j++) {
+                            const int *xa = g_xyz[ncomp_up_to(la - 1) + i], *xb = g_xyz[ncomp_up_to(lb - 1) + j];
+                            double s3[3], t3[3];
+                            for (int t = 0; t < 3; t++) {
+                                int u = xa[t], v = xb[t];
+                                s3[t] = Sx[t][u][v];
+                                t3[t] = 2.0 * al * be * Sx[t][u + 1][v + 1] +
+                                        (u > 0 && v > 0 ? 0.5 * u * v * Sx[t][u - 1][v - 1] : 0.0) -
+                                        (v > 0 ? al * v * Sx[t][u + 1][v - 1] : 0.0) -
+                                        (u > 0 ? be * u * Sx[t][u - 1][v + 1] : 0.0);
+                            }
+                            sb[i][j] += pre * s3[0] * s3[1] * s3[2];
+                            tb[i][j] += pre * (t3[0] * s3[1] * s3[2] + s3[0] * t3[1] * s3[2] +
+                                               s3[0] * s3[1] * t3[2]);
+                        }
+                    /* nuclear attraction */
+                    int L = la + lb, nr = L / 2 + 1;
+                    for (int C = 0; C < s->n_atom; C++) {
+                        const double *Cc = s->atom_xyz + 3 * C;
+                        double PC2 = 0;
+                        for (int t = 0; t < 3; t++) PC2 += (Pc[t] - Cc[t]) * (Pc[t] - Cc[t]);
+                        double u[16], w[16];
+                        rys(nr, p * PC2, u, w, 0);
+                        double vpre = -s->atom_Z[C] * 2.0 * M_PI / p * exp(-mu * AB2) * cc;
+                        for (int r = 0; r < nr; r++) {
+                            double I1[3][8][8];
+                            for (int t = 0; t < 3; t++) {
+                                double c00 = (Pc[t] - A[t]) + u[r] * (Cc[t] - Pc[t]);
+                                double b10 = (1.0 - u[r]) * 0.5 / p;
+                                double e1d[10];
+                                e1d[0] = 1.0;
+                                for (int e = 0; e < L; e++)
+                                    e1d[e + 1] = c00 * e1d[e] + (e ? e * b10 * e1d[e - 1] : 0.0);
+                                /* HRR (i, j+1) = (i+1, j) + (A-B)(i, j) */
+                                double H[10][10];
+                                for (int e = 0; e <= L; e++) H[e][0] = e1d[e];
+                                for (int j = 1; j <= lb; j++)
+                                    for (int i = 0; i <= L - j; i++)
+                                        H[i][j] = H[i + 1][j - 1] + (A[t] - B[t]) * H[i][j - 1];
+                                for (int i = 0; i <= la; i++)
+                                    for (int j = 0; j <= lb; j++) I1[t][i][j] = H[i][j];
+                            }
+                            for (int i = 0; i < na; i++)
+                                for (int j = 0; j < nb; j++) {
+                                    const int *xa = g_xyz[ncomp_up_to(la - 1) + i], *xb = g_xyz[ncomp_up_to(lb - 1) + j];
+                                    vb[i][j] += vpre * w[r] * I1[0][xa[0]][xb[0]] * I1[1][xa[1]][xb[1]] *
+                                                I1[2][xa[2]][xb[2]];
+                                }
+                        }
+                    }
+                }
+            int fa = s->first_bf[a], fb = s->first_bf[b];
+            for (int i = 0; i < na; i++)
+                for (int j = 0; j < nb; j++) {
+                    size_t ij = (size_t)(fa + i) * n + fb + j, ji = (size_t)(fb + j) * n + fa + i;
+                    S[ij] = S[ji] = sb[i][j];
+                    T[ij] = T[ji] = tb[i][j];
+                    V[ij] = V[ji] = vb[i][j];
+                }
+        }
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n); }

@@ -1,0 +1,153 @@
+"""ORACLE (test infrastructure only): ctypes wrapper of oracle/liboracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module,
+as the checker.  The product (tonto_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        d = ctypes.POINTER(ctypes.c_double)
+        _lib.orc_make_r_JK.argtypes = [ctypes.c_void_p, ctypes.c_void_p, d, ctypes.c_void_p, d, d,
+                                       ctypes.POINTER(ctypes.c_long)]
+        _lib.orc_eri_quartet.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + \
+            [ctypes.c_double, ctypes.c_int, d]
+        _lib.orc_eri_quartet.restype = ctypes.c_long
+        _lib.orc_schwarz.argtypes = [ctypes.c_void_p, ctypes.c_int, d]
+        _lib.orc_one_electron.argtypes = [ctypes.c_void_p, d, d, d]
+        _lib.orc_rys_table.argtypes = [ctypes.c_int, ctypes.c_double, d, d]
+        _lib.orq_rys.argtypes = [ctypes.c_int, ctypes.c_double, d, d]
+        _lib.orq_rys.restype = ctypes.c_int
+        _lib.orq_boys.argtypes = [ctypes.c_int, ctypes.c_double]
+        _lib.orq_boys.restype = ctypes.c_double
+        _lib.orc_set_threads.argtypes = [ctypes.c_int]
+        _lib.orc_max_threads.restype = ctypes.c_int
+        _lib.orc_component.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class _OSys(ctypes.Structure):
+    _fields_ = [("n_shell", ctypes.c_int), ("n_atom", ctypes.c_int), ("n_bf", ctypes.c_int)] + \
+        [(k, ctypes.POINTER(ctypes.c_int)) for k in ("l", "nprim", "atom", "first_bf", "prim_off")] + \
+        [(k, ctypes.POINTER(ctypes.c_double)) for k in ("exps", "coefs", "atom_xyz", "atom_Z")]
+
+
+class _OCut(ctypes.Structure):
+    _fields_ = [("schwarz", ctypes.c_double), ("jp", ctypes.c_double), ("kp", ctypes.c_double),
+                ("pair", ctypes.c_double), ("rys_quad", ctypes.c_int)]
+
+
+class OracleSystem:
+    """Holds a molecule's shell arrays alive for the C oracle."""
+
+    def __init__(self, mol):
+        self.mol = mol
+        self.arr = mol.shell_arrays()
+        a = self.arr
+        self.s = _OSys(mol.n_shell, mol.n_atom, mol.n_bf, _ip(a["l"]), _ip(a["nprim"]),
+                       _ip(a["atom"]), _ip(a["first_bf"]), _ip(a["prim_off"]), _dp(a["exps"]),
+                       _dp(a["coefs"]), _dp(a["atom_xyz"]), _dp(a["atom_Z"]))
+        self.n_bf = mol.n_bf
+        self.n_pairs = mol.n_shell * (mol.n_shell + 1) // 2
+
+    def schwarz(self, rys_quad=False):
+        out = np.zeros(self.n_pairs)
+        lib().orc_schwarz(ctypes.byref(self.s), int(rys_quad), _dp(out))
+        return out
+
+    def make_r_JK(self, P, schwarz=1e-9, jp=1e-9, kp=1e-9, pair=1e-6, max_I=None,
+                  rys_quad=False, return_stats=False):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n = self.n_bf
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        cut = _OCut(schwarz, jp, kp, pair, int(rys_quad))
+        st = (ctypes.c_long * 5)()
+        mi = None
+        if max_I is not None:
+            mi = np.ascontiguousarray(max_I, dtype=np.float64)
+        lib().orc_make_r_JK(ctypes.byref(self.s), ctypes.byref(cut), _dp(P),
+                            None if mi is None else mi.ctypes.data, _dp(J), _dp(K), st)
+        if return_stats:
+            return J, K, dict(quartets=st[0], prim_quartets=st[1], skip_J=st[2], skip_K=st[3],
+                              skip_lists=st[4])
+        return J, K
+
+    def eri_quartet(self, a, b, c, d, skip=False, pair=1e-6, rys_quad=False):
+        sh = self.mol.shells
+        dims = [sh[i].n_comp for i in (a, b, c, d)]
+        out = np.zeros(int(np.prod(dims)))
+        lib().orc_eri_quartet(ctypes.byref(self.s), a, b, c, d, int(skip), pair, int(rys_quad),
+                              _dp(out))
+        return out.reshape(dims)
+
+    def one_electron(self):
+        n = self.n_bf
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        lib().orc_one_electron(ctypes.byref(self.s), _dp(S), _dp(T), _dp(V))
+        return S, T, V
+
+    def full_eri(self, rys_quad=False):
+        """All (pq|rs) over basis functions, no screening/skipping (tiny systems only)."""
+        sh = self.mol.shells
+        n = self.n_bf
+        eri = np.zeros((n, n, n, n))
+        for a in range(len(sh)):
+            for b in range(len(sh)):
+                for c in range(len(sh)):
+                    for d in range(len(sh)):
+                        blk = self.eri_quartet(a, b, c, d, rys_quad=rys_quad)
+                        fa, fb, fc, fd = (sh[i].first_bf for i in (a, b, c, d))
+                        eri[fa:fa + blk.shape[0], fb:fb + blk.shape[1], fc:fc + blk.shape[2],
+                            fd:fd + blk.shape[3]] = blk
+        return eri
+
+
+def rys_table(n, X):
+    u, w = np.zeros(n), np.zeros(n)
+    lib().orc_rys_table(n, X, _dp(u), _dp(w))
+    return u, w
+
+
+def rys_quad(n, X):
+    u, w = np.zeros(n), np.zeros(n)
+    rc = lib().orq_rys(n, X, _dp(u), _dp(w))
+    assert rc == 0
+    return u, w
+
+
+def boys_quad(m, X):
+    return lib().orq_boys(m, X)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib().orc_max_threads()
