@@ -33,7 +33,8 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
-#include <omp.h>
+#include <pthread.h>
+#include <unistd.h>
 
 #include "../tonto_amd/csrc/rys_table.inc"
 
@@ -54,6 +55,52 @@ typedef struct {
     double schwarz, jp, kp, pair;   /* ERI_Schwarz_cutoff, ERI_J/K_density_cutoff, pair_cutoff */
     int rys_quad;                   /* 1: quad-precision Golub-Welsch roots */
 } ocut_t;
+
+/* ---------------- threads (plain pthreads: no OpenMP runtime, so the oracle can share a
+ * process with torch's own OpenMP copy) ------------------------------------------ */
+static int g_threads = 0;
+
+int orc_max_threads(void)
+{
+    if (g_threads <= 0) {
+        const char *e = getenv("ORACLE_THREADS");
+        g_threads = e ? atoi(e) : (int)sysconf(_SC_NPROCESSORS_ONLN);
+        if (g_threads <= 0) g_threads = 1;
+    }
+    return g_threads;
+}
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+typedef void (*work_fn)(void *arg, long item, int worker);
+typedef struct { work_fn fn; void *arg; long n; long next; pthread_mutex_t mu; int id; } pool_t;
+typedef struct { pool_t *pool; int worker; } wk_t;
+
+static void *pool_main(void *p)
+{
+    wk_t *w = (wk_t *)p;
+    for (;;) {
+        long item = __atomic_fetch_add(&w->pool->next, 1, __ATOMIC_RELAXED);
+        if (item >= w->pool->n) break;
+        w->pool->fn(w->pool->arg, item, w->worker);
+    }
+    return NULL;
+}
+
+/* run fn(arg, item, worker) for item in [0,n) on nthr workers, dynamic schedule */
+static void parallel_for(long n, int nthr, work_fn fn, void *arg)
+{
+    pool_t pool = {fn, arg, n, 0};
+    pthread_t th[256];
+    wk_t wk[256];
+    if (nthr > 256) nthr = 256;
+    for (int t = 0; t < nthr; t++) {
+        wk[t].pool = &pool;
+        wk[t].worker = t;
+        if (t > 0) pthread_create(&th[t], NULL, pool_main, &wk[t]);
+    }
+    pool_main(&wk[0]);
+    for (int t = 1; t < nthr; t++) pthread_join(th[t], NULL);
+}
 
 /* ---------------- Cartesian component tables (gaussian_data.foo:295-479) ----- */
 static int g_ready = 0;
@@ -226,12 +273,12 @@ static void hrr(int lbig, int lsm, const double *R, const double *in, int lead, 
 /* ---------------- one shell quartet (ab|cd) ------------------------------------- */
 /* out[((ia*nb + ib)*nc + ic)*nd + id]; returns number of primitive quartets */
 static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const ocut_t *cut,
-                    double *out)
+                    double *out, pairdat_t *ws)
 {
     comp_tables();
-    static __thread pairdat_t pab, pcd;
-    make_pair(s, a, b, skip, cut->pair, &pab);
-    make_pair(s, c, d, skip, cut->pair, &pcd);
+    pairdat_t *pab = &ws[0], *pcd = &ws[1];
+    make_pair(s, a, b, skip, cut->pair, pab);
+    make_pair(s, c, d, skip, cut->pair, pcd);
     int la = s->l[a], lb = s->l[b], lc = s->l[c], ld = s->l[d];
     int lab = la + lb, lcd = lc + ld;
     int lbig_ab = la > lb ? la : lb, lsm_ab = la > lb ? lb : la;
@@ -242,16 +289,16 @@ static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const
     double *G = calloc((size_t)ne * nf, sizeof(double));
     double Ix[9][9], Iy[9][9], Iz[9][9];
     double u[16], w[16];
-    for (int p = 0; p < pab.n; p++) {
-        double zeta = pab.zeta[p], zinv = 1.0 / zeta;
-        for (int q = 0; q < pcd.n; q++) {
-            double eta = pcd.zeta[q], einv = 1.0 / eta;
+    for (int p = 0; p < pab->n; p++) {
+        double zeta = pab->zeta[p], zinv = 1.0 / zeta;
+        for (int q = 0; q < pcd->n; q++) {
+            double eta = pcd->zeta[q], einv = 1.0 / eta;
             double rho = zeta * eta / (zeta + eta);
-            double QP[3] = {pcd.P[q][0] - pab.P[p][0], pcd.P[q][1] - pab.P[p][1],
-                            pcd.P[q][2] - pab.P[p][2]};
+            double QP[3] = {pcd->P[q][0] - pab->P[p][0], pcd->P[q][1] - pab->P[p][1],
+                            pcd->P[q][2] - pab->P[p][2]};
             double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
             rys(nroot, X, u, w, cut->rys_quad);
-            double pref = TWOPI5ON2 * pab.K[p] * pcd.K[q] * sqrt(rho);
+            double pref = TWOPI5ON2 * pab->K[p] * pcd->K[q] * sqrt(rho);
             for (int r = 0; r < nroot; r++) {
                 double t2 = u[r];
                 double rz = t2 * rho * zinv, re = t2 * rho * einv;
@@ -260,8 +307,8 @@ static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const
                 double (*I[3])[9] = {Ix, Iy, Iz};
                 for (int t = 0; t < 3; t++) {
                     double (*M)[9] = I[t];
-                    double c00 = pab.PA[p][t] + rz * QP[t];
-                    double d00 = pcd.PA[q][t] - re * QP[t];
+                    double c00 = pab->PA[p][t] + rz * QP[t];
+                    double d00 = pcd->PA[q][t] - re * QP[t];
                     M[0][0] = 1.0;
                     for (int e = 0; e < lab; e++)
                         M[e + 1][0] = c00 * M[e][0] + (e ? e * b10 * M[e - 1][0] : 0.0);
@@ -281,21 +328,21 @@ static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const
             }
         }
     }
-    long nprimq = (long)pab.n * pcd.n;
+    long nprimq = (long)pab->n * pcd->n;
     /* HRR on cd (trailing index), then ab */
     int nc = ncomp(lc), nd = ncomp(ld), na_ = ncomp(la), nb_ = ncomp(lb);
     int nbig_cd = ncomp(lbig_cd), nsm_cd = ncomp(lsm_cd);
     double *escd = malloc(sizeof(double) * ne * nbig_cd * nsm_cd);
-    hrr(lbig_cd, lsm_cd, pcd.R, G, ne, 1, escd);            /* [e][big][small] */
+    hrr(lbig_cd, lsm_cd, pcd->R, G, ne, 1, escd);            /* [e][big][small] */
     int nbig_ab = ncomp(lbig_ab), nsm_ab = ncomp(lsm_ab);
     double *abcd = malloc(sizeof(double) * nbig_ab * nsm_ab * nbig_cd * nsm_cd);
-    hrr(lbig_ab, lsm_ab, pab.R, escd, 1, nbig_cd * nsm_cd, abcd); /* [big][small][cdbig][cdsmall] */
+    hrr(lbig_ab, lsm_ab, pab->R, escd, 1, nbig_cd * nsm_cd, abcd); /* [big][small][cdbig][cdsmall] */
     for (int ia = 0; ia < na_; ia++)
         for (int ib = 0; ib < nb_; ib++)
             for (int ic = 0; ic < nc; ic++)
                 for (int id = 0; id < nd; id++) {
-                    int i1 = pab.big_is_first ? ia : ib, i2 = pab.big_is_first ? ib : ia;
-                    int i3 = pcd.big_is_first ? ic : id, i4 = pcd.big_is_first ? id : ic;
+                    int i1 = pab->big_is_first ? ia : ib, i2 = pab->big_is_first ? ib : ia;
+                    int i3 = pcd->big_is_first ? ic : id, i4 = pcd->big_is_first ? id : ic;
                     out[((ia * nb_ + ib) * nc + ic) * nd + id] =
                         abcd[((i1 * nsm_ab + i2) * nbig_cd + i3) * nsm_cd + i4];
                 }
@@ -308,19 +355,26 @@ long orc_eri_quartet(const osys_t *s, int a, int b, int c, int d, int skip, doub
                      int rys_quad, double *out)
 {
     ocut_t cut = {0, 0, 0, pair_cut, rys_quad};
-    return quartet(s, a, b, c, d, skip, &cut, out);
+    pairdat_t *ws = malloc(2 * sizeof(pairdat_t));
+    long n = quartet(s, a, b, c, d, skip, &cut, out, ws);
+    free(ws);
+    return n;
 }
 
 /* ---------------- Schwarz integrals (molecule.fock.foo:3712-3752) ---------------- */
-void orc_schwarz(const osys_t *s, int rys_quad, double *max_I)
+typedef struct { const osys_t *s; ocut_t cut; double *max_I; } schwarz_job;
+
+static void schwarz_item(void *arg, long item, int worker)
 {
-    int ns = s->n_shell;
-    ocut_t cut = {0, 0, 0, 1e-6, rys_quad};
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int a = 0; a < ns; a++) {
+    (void)worker;
+    schwarz_job *jb = (schwarz_job *)arg;
+    const osys_t *s = jb->s;
+    int a = (int)item;
+    {
         double *buf = malloc(sizeof(double) * 10 * 10 * 10 * 10);
+        pairdat_t *ws = malloc(2 * sizeof(pairdat_t));
         for (int b = 0; b <= a; b++) {
-            quartet(s, a, b, a, b, 0, &cut, buf);
+            quartet(s, a, b, a, b, 0, &jb->cut, buf, ws);
             int na = ncomp(s->l[a]), nb = ncomp(s->l[b]);
             double m = 0.0;
             for (int ia = 0; ia < na; ia++)
@@ -328,10 +382,18 @@ void orc_schwarz(const osys_t *s, int rys_quad, double *max_I)
                     double v = fabs(buf[((ia * nb + ib) * na + ia) * nb + ib]);
                     if (v > m) m = v;
                 }
-            max_I[a * (a + 1) / 2 + b] = sqrt(m);
+            jb->max_I[a * (a + 1) / 2 + b] = sqrt(m);
         }
+        free(ws);
         free(buf);
     }
+}
+
+void orc_schwarz(const osys_t *s, int rys_quad, double *max_I)
+{
+    comp_tables();
+    schwarz_job jb = {s, {0, 0, 0, 1e-6, rys_quad}, max_I};
+    parallel_for(s->n_shell, orc_max_threads(), schwarz_item, &jb);
 }
 
 /* ---------------- max density per shell pair (molecule.fock.foo:3601-3633) ------ */
@@ -352,11 +414,92 @@ static void max_density(const osys_t *s, const double *P, double *Pmax)
 
 static inline int pidx(int x, int y) { return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x; }
 
+typedef struct {
+    const osys_t *s;
+    const ocut_t *cut;
+    const double *P, *max_I, *Pmax;
+    double P_max, IP_max;
+    double **Jt, **Kt, **Ibuf;
+    pairdat_t **ws;
+    long (*st)[5];
+    long sample_stride;
+} jk_job;
+
+static void jk_item(void *arg, long ab, int w)
+{
+    jk_job *jb = (jk_job *)arg;
+    const osys_t *s = jb->s;
+    const ocut_t *cut = jb->cut;
+    const double *max_I = jb->max_I, *Pmax = jb->Pmax, *P = jb->P;
+    int n = s->n_bf;
+    double *Jl = jb->Jt[w], *Kl = jb->Kt[w], *I = jb->Ibuf[w];
+    long *st = jb->st[w];
+    if (max_I[ab] * jb->IP_max < cut->schwarz) return;
+    int a = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
+    while (a * (a + 1) / 2 > ab) a--;
+    while ((a + 1) * (a + 2) / 2 <= ab) a++;
+    int b = (int)(ab - a * (a + 1) / 2);
+    int atom_a = s->atom[a], atom_b = s->atom[b];
+    int ab_same = atom_a == atom_b;
+    for (long cd = 0; cd <= ab; cd++) {
+        if (max_I[ab] * max_I[cd] * jb->P_max < cut->schwarz) continue;
+        int c = (int)((sqrt(8.0 * cd + 1.0) - 1.0) / 2.0);
+        while (c * (c + 1) / 2 > cd) c--;
+        while ((c + 1) * (c + 2) / 2 <= cd) c++;
+        int d = (int)(cd - c * (c + 1) / 2);
+        int atom_c = s->atom[c], atom_d = s->atom[d];
+        int cd_same = atom_c == atom_d;
+        /* Schwarz_test */
+        double JP = Pmax[ab] > Pmax[cd] ? Pmax[ab] : Pmax[cd];
+        double KP = Pmax[pidx(a, c)];
+        if (Pmax[pidx(a, d)] > KP) KP = Pmax[pidx(a, d)];
+        if (Pmax[pidx(b, c)] > KP) KP = Pmax[pidx(b, c)];
+        if (Pmax[pidx(b, d)] > KP) KP = Pmax[pidx(b, d)];
+        int skip_J = (JP < cut->jp) || (JP * max_I[ab] * max_I[cd] < cut->schwarz);
+        int skip_K = (KP < cut->kp) || (KP * max_I[ab] * max_I[cd] < cut->schwarz);
+        if (skip_J && skip_K) continue;
+        double f = (a == b) ? 0.5 : 1.0;
+        if (c == d) f *= 0.5;
+        if (a == c && b == d) f *= 0.5;
+        int any_same = ab_same || cd_same || atom_a == atom_c || atom_a == atom_d ||
+                       atom_b == atom_c || atom_b == atom_d;
+        long npq = quartet(s, a, b, c, d, !any_same, cut, I, jb->ws[w]);
+        st[0]++; st[1] += npq; st[2] += skip_J; st[3] += skip_K; st[4] += !any_same;
+        int na = ncomp(s->l[a]), nb = ncomp(s->l[b]), nc = ncomp(s->l[c]), nd = ncomp(s->l[d]);
+        int fa = s->first_bf[a], fb = s->first_bf[b], fc = s->first_bf[c], fd = s->first_bf[d];
+        for (int ia = 0; ia < na; ia++)
+            for (int ib = 0; ib < nb; ib++)
+                for (int ic = 0; ic < nc; ic++)
+                    for (int id = 0; id < nd; id++) {
+                        double v = f * I[((ia * nb + ib) * nc + ic) * nd + id];
+                        int i = fa + ia, j = fb + ib, k = fc + ic, l = fd + id;
+                        if (!skip_J) {
+                            Jl[(size_t)i * n + j] += v * P[(size_t)k * n + l];
+                            Jl[(size_t)k * n + l] += v * P[(size_t)i * n + j];
+                        }
+                        if (!skip_K) {
+                            Kl[(size_t)i * n + k] += v * P[(size_t)l * n + j];
+                            Kl[(size_t)i * n + l] += v * P[(size_t)k * n + j];
+                            Kl[(size_t)j * n + k] += v * P[(size_t)l * n + i];
+                            Kl[(size_t)j * n + l] += v * P[(size_t)k * n + i];
+                        }
+                    }
+    }
+}
+
+static void jk_item_sample(void *arg, long item, int w)
+{
+    jk_job *jb = (jk_job *)arg;
+    jk_item(arg, item * jb->sample_stride, w);
+}
+
 /* ---------------- the J/K build (molecule.fock.foo:1365-1505) ------------------- */
 /* stats (may be NULL): [0] quartets computed, [1] primitive quartets, [2] J-skips,
  * [3] K-skips, [4] quartets using skipped primitive-pair lists */
-void orc_make_r_JK(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
-                   double *J, double *K, long *stats)
+static void jk_item_sample(void *arg, long item, int w);
+
+static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
+                           double *J, double *K, long *stats, long sample_stride)
 {
     comp_tables();
     int ns = s->n_shell, n = s->n_bf;
@@ -371,82 +514,37 @@ void orc_make_r_JK(const osys_t *s, const ocut_t *cut, const double *P, const do
         if (Pmax[i] > P_max) P_max = Pmax[i];
         if (max_I[i] > I_max) I_max = max_I[i];
     }
-    double IP_max = I_max * P_max;
+    int nthr = orc_max_threads();
+    jk_job jb;
+    jb.s = s; jb.cut = cut; jb.P = P; jb.max_I = max_I; jb.Pmax = Pmax;
+    jb.P_max = P_max; jb.IP_max = I_max * P_max;
+    jb.Jt = malloc(sizeof(double *) * nthr);
+    jb.Kt = malloc(sizeof(double *) * nthr);
+    jb.Ibuf = malloc(sizeof(double *) * nthr);
+    jb.ws = malloc(sizeof(pairdat_t *) * nthr);
+    jb.st = calloc(nthr, sizeof(long[5]));
+    for (int t = 0; t < nthr; t++) {
+        jb.Jt[t] = calloc((size_t)n * n, sizeof(double));
+        jb.Kt[t] = calloc((size_t)n * n, sizeof(double));
+        jb.Ibuf[t] = malloc(sizeof(double) * 10 * 10 * 10 * 10);
+        jb.ws[t] = malloc(2 * sizeof(pairdat_t));
+    }
+    if (sample_stride > 1) {
+        /* bounded sample of the same build: every sample_stride-th (ab| pair */
+        jb.sample_stride = sample_stride;
+        parallel_for((npair + sample_stride - 1) / sample_stride, nthr, jk_item_sample, &jb);
+    } else {
+        parallel_for(npair, nthr, jk_item, &jb);
+    }
     memset(J, 0, sizeof(double) * n * n);
     memset(K, 0, sizeof(double) * n * n);
     long st[5] = {0, 0, 0, 0, 0};
-    int nthr = omp_get_max_threads();
-    double **Jt = malloc(sizeof(double *) * nthr), **Kt = malloc(sizeof(double *) * nthr);
     for (int t = 0; t < nthr; t++) {
-        Jt[t] = calloc((size_t)n * n, sizeof(double));
-        Kt[t] = calloc((size_t)n * n, sizeof(double));
+        for (long i = 0; i < (long)n * n; i++) { J[i] += jb.Jt[t][i]; K[i] += jb.Kt[t][i]; }
+        for (int k = 0; k < 5; k++) st[k] += jb.st[t][k];
+        free(jb.Jt[t]); free(jb.Kt[t]); free(jb.Ibuf[t]); free(jb.ws[t]);
     }
-#pragma omp parallel reduction(+ : st[:5])
-    {
-        int tid = omp_get_thread_num();
-        double *Jl = Jt[tid], *Kl = Kt[tid];
-        double *I = malloc(sizeof(double) * 10 * 10 * 10 * 10);
-#pragma omp for schedule(dynamic, 1)
-        for (long ab = 0; ab < npair; ab++) {
-            if (max_I[ab] * IP_max < cut->schwarz) continue;
-            int a = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
-            while (a * (a + 1) / 2 > ab) a--;
-            while ((a + 1) * (a + 2) / 2 <= ab) a++;
-            int b = (int)(ab - a * (a + 1) / 2);
-            int atom_a = s->atom[a], atom_b = s->atom[b];
-            int ab_same = atom_a == atom_b;
-            for (long cd = 0; cd <= ab; cd++) {
-                if (max_I[ab] * max_I[cd] * P_max < cut->schwarz) continue;
-                int c = (int)((sqrt(8.0 * cd + 1.0) - 1.0) / 2.0);
-                while (c * (c + 1) / 2 > cd) c--;
-                while ((c + 1) * (c + 2) / 2 <= cd) c++;
-                int d = (int)(cd - c * (c + 1) / 2);
-                int atom_c = s->atom[c], atom_d = s->atom[d];
-                int cd_same = atom_c == atom_d;
-                /* Schwarz_test */
-                double JP = Pmax[ab] > Pmax[cd] ? Pmax[ab] : Pmax[cd];
-                double KP = Pmax[pidx(a, c)];
-                if (Pmax[pidx(a, d)] > KP) KP = Pmax[pidx(a, d)];
-                if (Pmax[pidx(b, c)] > KP) KP = Pmax[pidx(b, c)];
-                if (Pmax[pidx(b, d)] > KP) KP = Pmax[pidx(b, d)];
-                int skip_J = (JP < cut->jp) || (JP * max_I[ab] * max_I[cd] < cut->schwarz);
-                int skip_K = (KP < cut->kp) || (KP * max_I[ab] * max_I[cd] < cut->schwarz);
-                if (skip_J && skip_K) continue;
-                double f = (a == b) ? 0.5 : 1.0;
-                if (c == d) f *= 0.5;
-                if (a == c && b == d) f *= 0.5;
-                int any_same = ab_same || cd_same || atom_a == atom_c || atom_a == atom_d ||
-                               atom_b == atom_c || atom_b == atom_d;
-                long npq = quartet(s, a, b, c, d, !any_same, cut, I);
-                st[0]++; st[1] += npq; st[2] += skip_J; st[3] += skip_K; st[4] += !any_same;
-                int na = ncomp(s->l[a]), nb = ncomp(s->l[b]), nc = ncomp(s->l[c]), nd = ncomp(s->l[d]);
-                int fa = s->first_bf[a], fb = s->first_bf[b], fc = s->first_bf[c], fd = s->first_bf[d];
-                for (int ia = 0; ia < na; ia++)
-                    for (int ib = 0; ib < nb; ib++)
-                        for (int ic = 0; ic < nc; ic++)
-                            for (int id = 0; id < nd; id++) {
-                                double v = f * I[((ia * nb + ib) * nc + ic) * nd + id];
-                                int i = fa + ia, j = fb + ib, k = fc + ic, l = fd + id;
-                                if (!skip_J) {
-                                    Jl[(size_t)i * n + j] += v * P[(size_t)k * n + l];
-                                    Jl[(size_t)k * n + l] += v * P[(size_t)i * n + j];
-                                }
-                                if (!skip_K) {
-                                    Kl[(size_t)i * n + k] += v * P[(size_t)l * n + j];
-                                    Kl[(size_t)i * n + l] += v * P[(size_t)k * n + j];
-                                    Kl[(size_t)j * n + k] += v * P[(size_t)l * n + i];
-                                    Kl[(size_t)j * n + l] += v * P[(size_t)k * n + i];
-                                }
-                            }
-            }
-        }
-        free(I);
-    }
-    for (int t = 0; t < nthr; t++) {
-        for (long i = 0; i < (long)n * n; i++) { J[i] += Jt[t][i]; K[i] += Kt[t][i]; }
-        free(Jt[t]); free(Kt[t]);
-    }
-    free(Jt); free(Kt);
+    free(jb.Jt); free(jb.Kt); free(jb.Ibuf); free(jb.ws); free(jb.st);
     /* post-processing: weight_diagonal_blocks(J,2); blockwise_symmetric_fold(K); J = 2J;
      * symmetric_reflect (lower -> upper) */
     for (int a = 0; a < ns; a++) {
@@ -492,15 +590,16 @@ static void os1d(double PA, double PB, double p, int la, int lb, double S[][12])
         }
 }
 
-void orc_one_electron(const osys_t *s, double *S, double *T, double *V)
+typedef struct { const osys_t *s; double *S, *T, *V; } oe_job;
+
+static void oe_item(void *arg, long item, int worker)
 {
-    comp_tables();
-    int ns = s->n_shell, n = s->n_bf;
-    memset(S, 0, sizeof(double) * n * n);
-    memset(T, 0, sizeof(double) * n * n);
-    memset(V, 0, sizeof(double) * n * n);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int a = 0; a < ns; a++)
+    (void)worker;
+    oe_job *jb = (oe_job *)arg;
+    const osys_t *s = jb->s;
+    double *S = jb->S, *T = jb->T, *V = jb->V;
+    int n = s->n_bf, a = (int)item;
+    {
         for (int b = 0; b <= a; b++) {
             int la = s->l[a], lb = s->l[b], na = ncomp(la), nb = ncomp(lb);
             const double *A = s->atom_xyz + 3 * s->atom[a], *B = s->atom_xyz + 3 * s->atom[b];
@@ -580,7 +679,30 @@ void orc_one_electron(const osys_t *s, double *S, double *T, double *V)
                     V[ij] = V[ji] = vb[i][j];
                 }
         }
+    }
 }
 
-int orc_max_threads(void) { return omp_get_max_threads(); }
-void orc_set_threads(int n) { omp_set_num_threads(n); }
+void orc_one_electron(const osys_t *s, double *S, double *T, double *V)
+{
+    comp_tables();
+    int n = s->n_bf;
+    memset(S, 0, sizeof(double) * n * n);
+    memset(T, 0, sizeof(double) * n * n);
+    memset(V, 0, sizeof(double) * n * n);
+    oe_job jb = {s, S, T, V};
+    parallel_for(s->n_shell, orc_max_threads(), oe_item, &jb);
+}
+
+void orc_make_r_JK(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
+                   double *J, double *K, long *stats)
+{
+    make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, 1);
+}
+
+/* Timing sample for the CPU baseline: the same build restricted to every stride-th
+ * (ab| shell pair (J and K are then partial sums). */
+void orc_make_r_JK_sample(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
+                          double *J, double *K, long *stats, long stride)
+{
+    make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, stride < 1 ? 1 : stride);
+}

@@ -27,6 +27,7 @@ def lib():
         d = ctypes.POINTER(ctypes.c_double)
         _lib.orc_make_r_JK.argtypes = [ctypes.c_void_p, ctypes.c_void_p, d, ctypes.c_void_p, d, d,
                                        ctypes.POINTER(ctypes.c_long)]
+        _lib.orc_make_r_JK_sample.argtypes = _lib.orc_make_r_JK.argtypes + [ctypes.c_long]
         _lib.orc_eri_quartet.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + \
             [ctypes.c_double, ctypes.c_int, d]
         _lib.orc_eri_quartet.restype = ctypes.c_long
@@ -97,6 +98,19 @@ class OracleSystem:
             return J, K, dict(quartets=st[0], prim_quartets=st[1], skip_J=st[2], skip_K=st[3],
                               skip_lists=st[4])
         return J, K
+
+    def make_r_JK_sample(self, P, stride, schwarz=1e-9, jp=1e-9, kp=1e-9, pair=1e-6, max_I=None):
+        """Timing sample: the build over every stride-th (ab| pair; returns stats."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n = self.n_bf
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        cut = _OCut(schwarz, jp, kp, pair, 0)
+        st = (ctypes.c_long * 5)()
+        mi = None if max_I is None else np.ascontiguousarray(max_I, dtype=np.float64)
+        lib().orc_make_r_JK_sample(ctypes.byref(self.s), ctypes.byref(cut), _dp(P),
+                                   None if mi is None else mi.ctypes.data, _dp(J), _dp(K), st,
+                                   int(stride))
+        return dict(quartets=st[0], prim_quartets=st[1])
 
     def eri_quartet(self, a, b, c, d, skip=False, pair=1e-6, rys_quad=False):
         sh = self.mol.shells

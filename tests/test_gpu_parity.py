@@ -1,0 +1,152 @@
+"""GPU parity: libtonto_gpu (through its C ABI) against the CPU oracle on the same
+inputs.  Bars: max|dF| <= 1e-10 Eh with F = h + J - K/2 (so J and K to <= 1e-10 each
+on the scale used), the screening decisions identical (same number of quartets
+computed), the converged RHF energy within 1e-9 Eh of the oracle's."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible: GPU tests must run on an MI355X")
+    return torch
+
+
+def _rand_density(n, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n))
+    return scale * A @ A.T / n
+
+
+def _engine(mol, **kw):
+    from tonto_amd.gpu import JKEngine
+    return JKEngine(mol, **kw)
+
+
+def _check_jk(oracle_mod, mol, P, cut=1e-9, tol=1e-10, mixed=False):
+    o = oracle_mod.OracleSystem(mol)
+    e = _engine(mol, schwarz=cut)
+    mi = e.schwarz()
+    mo = o.schwarz()
+    assert np.max(np.abs(mi - mo) / np.maximum(mo, 1e-300)) < 1e-13
+    J, K = e.make_r_JK(P, mixed=mixed)
+    st = e.stats()
+    Jo, Ko, so = o.make_r_JK(P, schwarz=cut, max_I=mi, return_stats=True)
+    dJ, dK = np.max(np.abs(J - Jo)), np.max(np.abs(K - Ko))
+    dF = np.max(np.abs((J - 0.5 * K) - (Jo - 0.5 * Ko)))
+    print(f"{mol.n_bf} bf: dJ={dJ:.2e} dK={dK:.2e} dF={dF:.2e} quartets gpu={st['quartets']} "
+          f"oracle={so['quartets']}")
+    assert st["quartets"] == so["quartets"]
+    assert st["prim_quartets"] == so["prim_quartets"]
+    assert dF <= tol and dJ <= tol and dK <= tol
+    assert np.array_equal(J, J.T) and np.array_equal(K, K.T)
+    return e, o
+
+
+@pytest.mark.parametrize("basis", ["STO-3G", "6-31G(d)", "6-311G(d,p)", "cc-pVDZ", "DZP"])
+def test_water_all_classes(torch_cuda, oracle_mod, basis):
+    from tonto_amd.geometry import tonto_water
+    mol = tonto_water(basis)
+    _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 1))
+
+
+def test_four_distinct_atoms_skip_path(torch_cuda, oracle_mod):
+    from tonto_amd.geometry import cn_hcn_dimer
+    mol = cn_hcn_dimer()
+    _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 2, 0.2))
+
+
+def test_water_dimer_random_density_zero_cutoffs(torch_cuda, oracle_mod):
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(2, "6-31G(d,p)")
+    _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 3), cut=0.0)
+
+
+def test_config1_w8_converged_density(torch_cuda, oracle_mod):
+    """(H2O)_8 / 6-31G(d), 152 Cartesian functions, Schwarz cutoff 1e-12, at the
+    converged density (tests/golden/w8_631gd.npz)."""
+    from tonto_amd.geometry import water_cluster
+    g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))
+    mol = water_cluster(8, "6-31G(d)", seed=int(g["seed"]))
+    assert mol.n_bf == 152
+    _check_jk(oracle_mod, mol, g["P"], cut=1e-12)
+
+
+def test_w4_6311gdp_converged_density(torch_cuda, oracle_mod):
+    from tonto_amd.geometry import water_cluster
+    g = np.load(os.path.join(HERE, "golden", "w4_6311gdp.npz"))
+    mol = water_cluster(4, "6-311G(d,p)", seed=int(g["seed"]))
+    _check_jk(oracle_mod, mol, g["P"], cut=1e-9)
+
+
+def test_zero_density_gives_zero(torch_cuda):
+    from tonto_amd.geometry import tonto_water
+    mol = tonto_water("6-31G(d)")
+    e = _engine(mol)
+    J, K = e.make_r_JK(np.zeros((mol.n_bf, mol.n_bf)))
+    assert not J.any() and not K.any()
+    assert e.stats()["quartets"] == 0
+
+
+def test_single_atom_and_single_shell(torch_cuda, oracle_mod):
+    from tonto_amd.basis import Molecule
+    for mol in (Molecule.from_angstrom(["O"], [[0, 0, 0]], "cc-pVDZ"),
+                Molecule.from_angstrom(["H"], [[0, 0, 0]], "STO-3G")):
+        _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 4))
+
+
+def test_mixed_precision_error_is_small(torch_cuda, oracle_mod):
+    from tonto_amd.geometry import water_cluster
+    g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))
+    mol = water_cluster(8, "6-31G(d)", seed=int(g["seed"]))
+    e = _engine(mol, schwarz=1e-12, fp32_threshold=1e-6)
+    e.schwarz()
+    J, K = e.make_r_JK(g["P"])
+    Jm, Km = e.make_r_JK(g["P"], mixed=True)
+    st = e.stats()
+    assert st["fp32_quartets"] > 0
+    dF = np.max(np.abs((Jm - 0.5 * Km) - (J - 0.5 * K)))
+    print("mixed dF", dF, st)
+    assert dF < 1e-7
+
+
+def test_one_electron_parity(torch_cuda, oracle_mod):
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(2, "6-311G(d,p)")
+    e = _engine(mol)
+    S, T, V = e.one_electron()
+    So, To, Vo = oracle_mod.OracleSystem(mol).one_electron()
+    assert np.max(np.abs(S - So)) < 1e-12 and np.max(np.abs(T - To)) < 1e-11
+    assert np.max(np.abs(V - Vo)) < 1e-11
+
+
+def test_device_entry_point_matches_host(torch_cuda):
+    import torch
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(3, "6-31G(d)")
+    e = _engine(mol)
+    P = _rand_density(mol.n_bf, 5)
+    J, K = e.make_r_JK(P)
+    dP = torch.tensor(P, device="cuda")
+    dJ = torch.empty_like(dP)
+    dK = torch.empty_like(dP)
+    e.set_stream(torch.cuda.current_stream().cuda_stream)
+    e.make_r_JK_device(dP.data_ptr(), dJ.data_ptr(), dK.data_ptr())
+    torch.cuda.synchronize()
+    assert np.max(np.abs(dJ.cpu().numpy() - J)) < 1e-13 and np.max(np.abs(dK.cpu().numpy() - K)) < 1e-13
+
+
+def test_gpu_scf_reaches_g09(torch_cuda):
+    """H2O RHF/cc-pVDZ on the GPU path: g09 -76.0232731206 (scripts/check_dft_reference.py:67)."""
+    from tonto_amd.geometry import tonto_water
+    from tonto_amd.driver import GPUSCF
+    r = GPUSCF(tonto_water("cc-pVDZ")).run(convergence=1e-10, diis_tol=1e-7)
+    assert r.converged and abs(r.energy - (-76.0232731206)) < 1e-9

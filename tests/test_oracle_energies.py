@@ -1,0 +1,79 @@
+"""Whole-path pins of the oracle: converged RHF energies against the external g09
+number and Tonto's stored test outputs.
+
+* H2O RHF/cc-pVDZ (6D) = -76.0232731206 Eh from g09
+  (scripts/check_dft_reference.py:67, checked there at TOL_HF = 1e-8; Tonto's own value
+  -76.023273121245, docs/DFT_STANDARDISATION.md:110-118).
+* tests/short/h2o_rhf_*/stdout and tests/long/cn-hcn_dimer_rhf_cc-pVDZ_*/stdout print
+  energies to 4 decimals.
+The same geometry as those jobs (Angstrom) and Tonto's basis library are used."""
+import numpy as np
+import pytest
+import torch
+
+from tonto_amd.geometry import cn_hcn_dimer, tonto_water
+from tonto_amd.scf import run_rhf
+
+
+def oracle_scf(O, mol, conv=1e-10, tol=1e-7, **cut):
+    o = O.OracleSystem(mol)
+    S, T, V = o.one_electron()
+    maxI = o.schwarz()
+
+    def jk(P, mode):
+        J, K = o.make_r_JK(P.numpy(), max_I=maxI, **cut)
+        return torch.from_numpy(J), torch.from_numpy(K)
+    return run_rhf(torch.from_numpy(T + V), torch.from_numpy(S), mol.n_electrons,
+                   mol.nuclear_repulsion(), jk, convergence=conv, diis_tol=tol)
+
+
+def test_g09_h2o_cc_pvdz(oracle_mod):
+    r = oracle_scf(oracle_mod, tonto_water("cc-pVDZ"))
+    assert r.converged
+    assert abs(r.energy - (-76.0232731206)) < 1e-9
+    assert abs(r.energy - (-76.023273121245)) < 1e-9   # Tonto's printed 12-digit value
+
+
+@pytest.mark.parametrize("basis,ref", [("STO-3G", -74.9658), ("6-31G(d)", -76.0067),
+                                       ("6-311G(d,p)", -76.0419), ("DZP", -76.0420),
+                                       ("cc-pVDZ", -76.0233)])
+def test_tonto_stored_water_energies(oracle_mod, basis, ref):
+    r = oracle_scf(oracle_mod, tonto_water(basis))
+    assert r.converged
+    assert round(r.energy, 4) == pytest.approx(ref, abs=1e-9)
+
+
+def test_tonto_nuclear_repulsion():
+    # tests/short/h2o_rhf_6-31G(d)/stdout:190  V_NN = 8.8805
+    assert round(tonto_water("6-31G(d)").nuclear_repulsion(), 4) == 8.8805
+
+
+def test_cn_hcn_dimer_skip_path(oracle_mod):
+    """Five distinct atoms: the 1e-6 primitive-pair skip (shell1quartet.foo:229-273) is
+    live here; Tonto prints -185.2238 (tests/long/cn-hcn_dimer_.../stdout:262)."""
+    mol = cn_hcn_dimer()
+    o = oracle_mod.OracleSystem(mol)
+    r = oracle_scf(oracle_mod, mol, conv=1e-9, tol=1e-6)
+    assert r.converged
+    assert round(r.energy, 4) == pytest.approx(-185.2238, abs=1e-9)
+    _, _, st = o.make_r_JK(r.P.numpy(), return_stats=True)
+    assert st["skip_lists"] > 0
+
+
+def test_guess_independence(oracle_mod):
+    """scripts/check_single_atom_scf.py:54: the converged energy does not depend on the
+    initial guess (1e-8)."""
+    mol = tonto_water("6-31G(d)")
+    r1 = oracle_scf(oracle_mod, mol)
+    o = oracle_mod.OracleSystem(mol)
+    S, T, V = o.one_electron()
+    n = mol.n_bf
+    P0 = torch.eye(n, dtype=torch.float64) * (mol.n_electrons / n)
+    maxI = o.schwarz()
+
+    def jk(P, mode):
+        J, K = o.make_r_JK(P.numpy(), max_I=maxI)
+        return torch.from_numpy(J), torch.from_numpy(K)
+    r2 = run_rhf(torch.from_numpy(T + V), torch.from_numpy(S), mol.n_electrons,
+                 mol.nuclear_repulsion(), jk, P0=P0, convergence=1e-10, diis_tol=1e-7)
+    assert abs(r1.energy - r2.energy) < 1e-8

@@ -1,0 +1,104 @@
+"""Pins of the oracle's contracted ERIs (Rys VRR + HRR restatement of
+foofiles/shell1quartet.foo) against a closed form, a textbook known answer and an
+independent McMurchie-Davidson brute force; plus the symmetries the integrals obey."""
+import math
+
+import numpy as np
+import pytest
+
+from tonto_amd.basis import Molecule
+from tests import md_eri
+
+
+def test_ssss_closed_form(oracle_mod):
+    # one s primitive per centre: (ss|ss) = 2 pi^5/2 / (p q sqrt(p+q)) K_ab K_cd F0(T)
+    m = Molecule.from_angstrom(["H", "H"], [[0, 0, 0], [0.3, -0.2, 0.9]], "6-31G(d)")
+    o = oracle_mod.OracleSystem(m)
+    sa, sb = m.shells[1], m.shells[3]            # the single-primitive s shells
+    a, b = sa.exps[0], sb.exps[0]
+    A, B = np.array(sa.center), np.array(sb.center)
+    p = a + b
+    P = (a * A + b * B) / p
+    Kab = math.exp(-a * b / p * np.dot(A - B, A - B))
+    q, Q, Kcd = 2 * a, A, 1.0
+    T = p * q / (p + q) * np.dot(P - Q, P - Q)
+    F0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    ref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * F0 * \
+        sa.coefs[0] ** 3 * sb.coefs[0]
+    got = o.eri_quartet(1, 3, 1, 1)[0, 0, 0, 0]
+    assert abs(got - ref) < 1e-14 * abs(ref)
+
+
+def test_h2_sto3g_szabo_ostlund(oracle_mod):
+    # Szabo & Ostlund, Modern Quantum Chemistry, section 3.5.2: H2/STO-3G, R = 1.4 bohr
+    m = Molecule.from_angstrom(["H", "H"], [[0, 0, 0], [0, 0, 1.4 * 0.5291772108]], "STO-3G")
+    o = oracle_mod.OracleSystem(m)
+    e = o.full_eri()
+    assert abs(e[0, 0, 0, 0] - 0.7746) < 1e-4
+    assert abs(e[0, 0, 1, 1] - 0.5697) < 1e-4
+    assert abs(e[0, 1, 0, 1] - 0.2970) < 1e-4
+    assert abs(e[0, 0, 0, 1] - 0.4441) < 1e-4
+    S, T, V = o.one_electron()
+    assert abs(S[0, 1] - 0.6593) < 1e-4
+    assert abs(T[0, 0] - 0.7600) < 1e-4 and abs(T[0, 1] - 0.2365) < 1e-4
+    assert abs(V[0, 0] - (-1.2266 - 0.6538)) < 2e-4 and abs(V[0, 1] - (-1.1948)) < 1e-4
+
+
+def _four_centre_system():
+    # O with s,p,d shells plus three H-like centres (6-31G(d)): four distinct centres
+    return Molecule.from_angstrom(["O", "H", "O", "H"],
+                                  [[0.0, 0.0, 0.0], [0.7, 0.5, -0.2], [-0.4, 1.1, 0.6],
+                                   [0.9, -0.8, 0.7]], "6-31G(d)")
+
+
+def _pick(m, l, atom):
+    return next(i for i, s in enumerate(m.shells) if s.l == l and s.atom == atom)
+
+
+@pytest.mark.parametrize("ls", [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (1, 1, 1, 1), (2, 0, 1, 0),
+                                (2, 1, 0, 1), (2, 2, 1, 0), (2, 1, 2, 1), (2, 2, 2, 2), (0, 2, 1, 2)])
+def test_against_mcmurchie_davidson(oracle_mod, ls):
+    m = _four_centre_system()
+    o = oracle_mod.OracleSystem(m)
+    atoms = [0, 1, 2, 3]
+    idx = []
+    for l, at in zip(ls, atoms):
+        at = at if l == 0 or m.symbols[at] == "O" else [0, 2][at % 2]
+        idx.append(_pick(m, l, at))
+    # the last primitive (single-primitive shells) keeps the brute force cheap
+    got = o.eri_quartet(*idx)
+    ref = md_eri.shell_quartet(m.shells, *idx)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) < 1e-12 * scale, (ls, idx)
+
+
+def test_permutational_symmetry(oracle_mod):
+    m = _four_centre_system()
+    o = oracle_mod.OracleSystem(m)
+    a, b, c, d = _pick(m, 2, 0), _pick(m, 1, 2), _pick(m, 1, 0), _pick(m, 0, 3)
+    I = o.eri_quartet(a, b, c, d)
+    assert np.allclose(I, o.eri_quartet(b, a, c, d).transpose(1, 0, 2, 3), rtol=0, atol=1e-13)
+    assert np.allclose(I, o.eri_quartet(a, b, d, c).transpose(0, 1, 3, 2), rtol=0, atol=1e-13)
+    assert np.allclose(I, o.eri_quartet(c, d, a, b).transpose(2, 3, 0, 1), rtol=0, atol=1e-13)
+
+
+def test_schwarz_inequality_and_positivity(oracle_mod):
+    m = Molecule.from_angstrom(["O", "H", "H"], [[0, 0, 0], [0.76, 0.59, 0], [-0.76, 0.59, 0]],
+                               "6-31G(d)")
+    o = oracle_mod.OracleSystem(m)
+    eri = o.full_eri()
+    n = m.n_bf
+    M = eri.reshape(n * n, n * n)
+    assert np.allclose(M, M.T, atol=1e-12)
+    assert np.linalg.eigvalsh(M).min() > -1e-10          # (ab|cd) supermatrix is PSD
+    d = np.sqrt(np.abs(np.einsum("abab->ab", eri)))
+    assert np.all(np.abs(eri) <= d[:, :, None, None] * d[None, None, :, :] * (1 + 1e-12) + 1e-14)
+
+
+def test_rys_quad_and_table_integrals_agree(oracle_mod):
+    m = _four_centre_system()
+    o = oracle_mod.OracleSystem(m)
+    q = (_pick(m, 2, 0), _pick(m, 2, 2), _pick(m, 1, 0), _pick(m, 2, 2))
+    a = o.eri_quartet(*q)
+    b = o.eri_quartet(*q, rys_quad=True)
+    assert np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(b))

@@ -143,6 +143,9 @@ def build(check=True):
 def write_inc(path, out, herm):
     lines = ["/* Generated by gen_rys_table.py: Chebyshev tables of Rys roots (u = t^2) and",
              " * weights, and the Gauss-Hermite large-X limit.  Do not edit. */",
+             "#ifndef RYS_QUAL",
+             "#define RYS_QUAL static const",
+             "#endif",
              f"#define RYS_NMAX {NMAX}",
              f"#define RYS_NCHEB {NCHEB}",
              f"#define RYS_H {H!r}",
@@ -158,9 +161,9 @@ def write_inc(path, out, herm):
                     allc.append(v)
         off += len(out[n]) * NCHEB * 2 * n
     lines.append(f"#define RYS_TABLE_SIZE {off}")
-    lines.append("static const int rys_table_offset[RYS_NMAX + 1] = {0, "
+    lines.append("RYS_QUAL int rys_table_offset[RYS_NMAX + 1] = {0, "
                  + ", ".join(str(o) for o in offs) + "};")
-    lines.append("static const double rys_xasym[RYS_NMAX + 1] = {0.0, "
+    lines.append("RYS_QUAL double rys_xasym[RYS_NMAX + 1] = {0.0, "
                  + ", ".join(repr(float(XASYM[n])) for n in range(1, NMAX + 1)) + "};")
     hs = []
     hw = []
@@ -169,10 +172,10 @@ def write_inc(path, out, herm):
         hs += [mp.nstr(v * v, 20, strip_zeros=False) for v in s] + ["0.0"] * (NMAX - n)
         hw += [mp.nstr(v, 20, strip_zeros=False) for v in W] + ["0.0"] * (NMAX - n)
     lines.append("/* [n-1][i]: s_i^2 and W_i of the positive 2n-point Gauss-Hermite nodes */")
-    lines.append("static const double rys_herm_s2[RYS_NMAX * RYS_NMAX] = {" + ", ".join(hs) + "};")
-    lines.append("static const double rys_herm_w[RYS_NMAX * RYS_NMAX] = {" + ", ".join(hw) + "};")
+    lines.append("RYS_QUAL double rys_herm_s2[RYS_NMAX * RYS_NMAX] = {" + ", ".join(hs) + "};")
+    lines.append("RYS_QUAL double rys_herm_w[RYS_NMAX * RYS_NMAX] = {" + ", ".join(hw) + "};")
     lines.append("/* layout: for n, interval k, coefficient m: 2n values (n roots then n weights) */")
-    lines.append("static const double rys_table_data[RYS_TABLE_SIZE] = {")
+    lines.append("RYS_QUAL double rys_table_data[RYS_TABLE_SIZE] = {")
     for i in range(0, len(allc), 4):
         lines.append("  " + ", ".join(mp.nstr(v, 20, strip_zeros=False) for v in allc[i:i + 4]) + ",")
     lines.append("};")

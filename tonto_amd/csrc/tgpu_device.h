@@ -1,0 +1,624 @@
+// Device-side building blocks shared by the per-class translation units and the host
+// translation unit of libtonto_gpu (see tgpu_host.hip for the data layout).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/tonto_gpu.h"
+#include "gen/classes.inc"
+
+#define RYS_QUAL static __device__ const
+#include "rys_table.inc"
+
+#define TWOPI5ON2 34.9868366552497256925  // include/macros.in:446
+
+// ----------------------------------------------------------------------------------
+// Device helpers
+// ----------------------------------------------------------------------------------
+struct PairView {
+    const double *zeta, *K, *Px, *Py, *Pz;
+    int n;
+    double Ax, Ay, Az, ABx, ABy, ABz;
+};
+
+// Rys roots (u = t^2) and weights, FP64: Chebyshev pieces of width RYS_H below
+// rys_xasym[N], Gauss-Hermite limit above (table made by gen_rys_table.py).
+template <int N>
+__device__ __forceinline__ void rys_roots(double X, double *u, double *w)
+{
+    if (X >= rys_xasym[N]) {
+        const double ix = 1.0 / X, isx = sqrt(ix);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            u[i] = rys_herm_s2[(N - 1) * RYS_NMAX + i] * ix;
+            w[i] = rys_herm_w[(N - 1) * RYS_NMAX + i] * isx;
+        }
+        return;
+    }
+    const int k = (int)(X * (1.0 / RYS_H));
+    const double t = (X - k * RYS_H) * (2.0 / RYS_H) - 1.0, tt = 2.0 * t;
+    const double *c = rys_table_data + rys_table_offset[N] + k * (RYS_NCHEB * 2 * N);
+    double b1[2 * N], b2[2 * N];
+#pragma unroll
+    for (int f = 0; f < 2 * N; ++f) {
+        b1[f] = 0.0;
+        b2[f] = 0.0;
+    }
+#pragma unroll
+    for (int m = RYS_NCHEB - 1; m >= 1; --m) {
+#pragma unroll
+        for (int f = 0; f < 2 * N; ++f) {
+            const double b0 = tt * b1[f] - b2[f] + c[m * 2 * N + f];
+            b2[f] = b1[f];
+            b1[f] = b0;
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+        u[f] = t * b1[f] - b2[f] + c[f];
+        w[f] = t * b1[N + f] - b2[N + f] + c[N + f];
+    }
+}
+
+static __device__ void rys_roots_rt(int n, double X, double *u, double *w)
+{
+    switch (n) {
+    case 1: rys_roots<1>(X, u, w); break;
+    case 2: rys_roots<2>(X, u, w); break;
+    case 3: rys_roots<3>(X, u, w); break;
+    case 4: rys_roots<4>(X, u, w); break;
+    case 5: rys_roots<5>(X, u, w); break;
+    case 6: rys_roots<6>(X, u, w); break;
+    default: rys_roots<7>(X, u, w); break;
+    }
+}
+
+static constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+static constexpr int pair_class(int lp, int lq) { return lp * (lp + 1) / 2 + lq; }
+
+template <int LP, int LQ, int LR, int LS, typename R> struct EriBody;
+template <int LP, int LQ, int LR, int LS> struct WaveTab;
+
+struct DevPairs {
+    const int *tonto, *sh_p, *sh_q, *bf_p, *bf_q, *atom_p, *atom_q;
+    const int *off_full, *n_full, *off_skip, *n_skip;
+    const double *A, *AB, *maxI;
+};
+struct DevPrims {
+    const double *zeta, *K, *Px, *Py, *Pz;
+};
+
+__device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims &pm, int s,
+                                              bool skip)
+{
+    PairView v;
+    const int off = skip ? pr.off_skip[s] : pr.off_full[s];
+    v.n = skip ? pr.n_skip[s] : pr.n_full[s];
+    v.zeta = pm.zeta + off;
+    v.K = pm.K + off;
+    v.Px = pm.Px + off;
+    v.Py = pm.Py + off;
+    v.Pz = pm.Pz + off;
+    v.Ax = pr.A[3 * s];
+    v.Ay = pr.A[3 * s + 1];
+    v.Az = pr.A[3 * s + 2];
+    v.ABx = pr.AB[3 * s];
+    v.ABy = pr.AB[3 * s + 1];
+    v.ABz = pr.AB[3 * s + 2];
+    return v;
+}
+
+__device__ __forceinline__ long long tonto_pair(int x, int y)
+{
+    return x >= y ? (long long)x * (x + 1) / 2 + y : (long long)y * (y + 1) / 2 + x;
+}
+
+struct JKArgs {
+    DevPairs pr;
+    DevPrims pm;
+    int bra_start, n_bra, ket_start, n_ket, same_bin, nbf, counter_slot;
+    const double *P, *Pmax, *scal;
+    double cutoff, jp, kp, fp32_thr;
+    double *X, *Y;
+    unsigned long long *counters;
+};
+
+// ----------------------------------------------------------------------------------
+// The J/K kernel for one class pair (LP LQ | LR LS)
+// ----------------------------------------------------------------------------------
+template <int LP, int LQ, int LR, int LS, bool MIXED>
+__global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
+{
+    constexpr int NP = ncart(LP), NQ = ncart(LQ), NR = ncart(LR), NS = ncart(LS);
+    constexpr int NPQ = NP * NQ, NRS = NR * NS, NI = NPQ * NRS;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n_bra) return;  // wave-uniform
+    const int sb = a.bra_start + i;
+    const int nbf = a.nbf;
+    const int tb = a.pr.tonto[sb];
+    const double mIb = a.pr.maxI[sb];
+    const int bp = a.pr.sh_p[sb], bq = a.pr.sh_q[sb];
+    const int atp = a.pr.atom_p[sb], atq = a.pr.atom_q[sb];
+    const int fp = a.pr.bf_p[sb], fq = a.pr.bf_q[sb];
+    const double Pmax_b = a.Pmax[tb];
+    const double P_max = a.scal[0], IP_max = a.scal[1];
+    const double *__restrict__ P = a.P;
+    const double *__restrict__ maxI = a.pr.maxI + a.ket_start;
+
+    // Schwarz frontier: kets are sorted by max_I descending, so the kets passing
+    // max_I(ab)*max_I(cd)*P_max >= cutoff (molecule.fock.foo:1430) form a prefix.
+    const int jlim = a.same_bin ? i + 1 : a.n_ket;
+    int lo = 0, hi = jlim;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (mIb * maxI[mid] * P_max < a.cutoff) hi = mid;
+        else lo = mid + 1;
+    }
+    const int jend = lo;
+    int jsplit = jend;
+    if (MIXED) {
+        lo = 0;
+        hi = jend;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (mIb * maxI[mid] < a.fp32_thr) hi = mid;
+            else lo = mid + 1;
+        }
+        jsplit = lo;
+    }
+
+    double Ppq[NPQ], Jpq[NPQ];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            Ppq[p * NQ + q] = P[(size_t)(fp + p) * nbf + fq + q];
+            Jpq[p * NQ + q] = 0.0;
+        }
+    unsigned long long nq = 0, npq = 0, n32 = 0;
+
+    for (int j0 = 0; j0 < jend; j0 += 64) {
+        const int j = j0 + lane;
+        if (j >= jend) continue;
+        const int sk = a.ket_start + j;
+        const int tk = a.pr.tonto[sk];
+        const double mIk = maxI[j];
+        const bool bhi = tb >= tk;  // Tonto's (ab| is the pair with the larger index
+        const double Iab = bhi ? mIb : mIk, Icd = bhi ? mIk : mIb;
+        if (Iab * IP_max < a.cutoff) continue;  // molecule.fock.foo:1412
+        const int kr = a.pr.sh_p[sk], ks = a.pr.sh_q[sk];
+        const double Pmax_k = a.Pmax[tk];
+        const double JP = bhi ? fmax(Pmax_b, Pmax_k) : fmax(Pmax_k, Pmax_b);
+        const double KP = fmax(fmax(a.Pmax[tonto_pair(bp, kr)], a.Pmax[tonto_pair(bp, ks)]),
+                               fmax(a.Pmax[tonto_pair(bq, kr)], a.Pmax[tonto_pair(bq, ks)]));
+        const bool skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
+        const bool skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
+        if (skipJ && skipK) continue;
+        double f = (bp == bq) ? 0.5 : 1.0;
+        if (kr == ks) f *= 0.5;
+        if (sb == sk) f *= 0.5;
+        const int akr = a.pr.atom_p[sk], aks = a.pr.atom_q[sk];
+        const bool any_same = atp == atq || akr == aks || atp == akr || atp == aks ||
+                              atq == akr || atq == aks;
+        const PairView bv = make_view(a.pr, a.pm, sb, !any_same);
+        const PairView kv = make_view(a.pr, a.pm, sk, !any_same);
+        double I[NI];
+        if (!MIXED || j < jsplit) {
+            EriBody<LP, LQ, LR, LS, double>::run(bv, kv, I);
+        } else {
+            EriBody<LP, LQ, LR, LS, float>::run(bv, kv, I);
+            ++n32;
+        }
+        ++nq;
+        npq += (unsigned long long)(bv.n * kv.n);
+        const int fr = a.pr.bf_p[sk], fs = a.pr.bf_q[sk];
+        if (!skipJ) {
+            double Prs[NRS];
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) Prs[r * NS + s] = P[(size_t)(fr + r) * nbf + fs + s];
+#pragma unroll
+            for (int rs = 0; rs < NRS; ++rs) {
+                double acc = 0.0;
+#pragma unroll
+                for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + rs] * Ppq[pq];
+                unsafeAtomicAdd(&a.X[(size_t)(fr + rs / NS) * nbf + fs + rs % NS], f * acc);
+            }
+#pragma unroll
+            for (int pq = 0; pq < NPQ; ++pq) {
+                double acc = 0.0;
+#pragma unroll
+                for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * Prs[rs];
+                Jpq[pq] += f * acc;
+            }
+        }
+        if (!skipK) {
+            // K(p,r) += I P(s,q); K(p,s) += I P(r,q); K(q,r) += I P(s,p); K(q,s) += I P(r,p)
+            // (shell1quartet.foo:11508-11521)
+            double Psq[NS * NQ], Prq[NR * NQ], Psp[NS * NP], Prp[NR * NP];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fs + s) * nbf + fq + q];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fr + r) * nbf + fq + q];
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fs + s) * nbf + fp + p];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fr + r) * nbf + fp + p];
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                        for (int s = 0; s < NS; ++s)
+                            acc += I[((p * NQ + q) * NR + r) * NS + s] * Psq[s * NQ + q];
+                    unsafeAtomicAdd(&a.Y[(size_t)(fp + p) * nbf + fr + r], f * acc);
+                }
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r)
+                            acc += I[((p * NQ + q) * NR + r) * NS + s] * Prq[r * NQ + q];
+                    unsafeAtomicAdd(&a.Y[(size_t)(fp + p) * nbf + fs + s], f * acc);
+                }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p)
+#pragma unroll
+                        for (int s = 0; s < NS; ++s)
+                            acc += I[((p * NQ + q) * NR + r) * NS + s] * Psp[s * NP + p];
+                    unsafeAtomicAdd(&a.Y[(size_t)(fq + q) * nbf + fr + r], f * acc);
+                }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r)
+                            acc += I[((p * NQ + q) * NR + r) * NS + s] * Prp[r * NP + p];
+                    unsafeAtomicAdd(&a.Y[(size_t)(fq + q) * nbf + fs + s], f * acc);
+                }
+        }
+    }
+    // wavefront reduction of the bra-pair J block
+#pragma unroll
+    for (int pq = 0; pq < NPQ; ++pq) {
+        double v = Jpq[pq];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0 && v != 0.0)
+            unsafeAtomicAdd(&a.X[(size_t)(fp + pq / NQ) * nbf + fq + pq % NQ], v);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        nq += __shfl_xor(nq, off, 64);
+        npq += __shfl_xor(npq, off, 64);
+        n32 += __shfl_xor(n32, off, 64);
+    }
+    if (lane == 0 && nq) {
+        atomicAdd(&a.counters[3 * a.counter_slot + 0], nq);
+        atomicAdd(&a.counters[3 * a.counter_slot + 1], npq);
+        atomicAdd(&a.counters[3 * a.counter_slot + 2], n32);
+    }
+}
+
+// Schwarz table: max_I = sqrt(max |(ab|ab)|) with the full primitive lists
+// (molecule.fock.foo:3712-3752).  One thread per shell pair of one class bin.
+template <int LP, int LQ>
+__global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, int start, int n,
+                                                    double *maxI)
+{
+    constexpr int NP = ncart(LP), NQ = ncart(LQ);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = start + i;
+    const PairView v = make_view(pr, pm, s, false);
+    double I[NP * NQ * NP * NQ];
+    EriBody<LP, LQ, LP, LQ, double>::run(v, v, I);
+    double m = 0.0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) m = fmax(m, fabs(I[((p * NQ + q) * NP + p) * NQ + q]));
+    maxI[s] = sqrt(m);
+}
+
+
+// ----------------------------------------------------------------------------------
+// Heavy classes: one 64-lane wavefront (= one work-group) per quartet.
+// Per primitive quartet, lanes (axis, root) build the 2-D Rys tables into LDS and every
+// lane accumulates its slice of the (e0|f0) entries in registers; the ket-side and then
+// the bra-side transfer relation run level by level in LDS with lanes over entries;
+// the digestion gives each J/K target element to one lane (one atomic per element).
+// ----------------------------------------------------------------------------------
+template <int LP, int LQ, int LR, int LS>
+struct WaveLDS {
+    using T = WaveTab<LP, LQ, LR, LS>;
+    double g[3 * T::NROOT * (T::LPQ + 1) * (T::LRS + 1)];
+    double buf[2][T::BUF];
+    double Ppq[T::NP * T::NQ], Prs[T::NR * T::NS];
+    double Psq[T::NS * T::NQ], Prq[T::NR * T::NQ], Psp[T::NS * T::NP], Prp[T::NR * T::NP];
+};
+
+// Computes (pq|rs) of one contracted quartet into LDS; returns the buffer holding
+// [pq][rs].  Must be called by all 64 lanes of the work-group.
+template <int LP, int LQ, int LR, int LS>
+__device__ const double *wave_eri(const PairView &bra, const PairView &ket, WaveLDS<LP, LQ, LR, LS> &L)
+{
+    using T = WaveTab<LP, LQ, LR, LS>;
+    constexpr int N = T::NROOT, W1 = T::LPQ + 1, W2 = T::LRS + 1, TW = W1 * W2;
+    constexpr int NG = T::NE * T::NF, MG = (NG + 63) / 64;
+    constexpr int NRS = T::NR * T::NS;
+    const int lane = threadIdx.x;
+    double Gr[MG];
+    int gi[MG][3];
+#pragma unroll
+    for (int m = 0; m < MG; ++m) {
+        Gr[m] = 0.0;
+        const int k = lane + 64 * m;
+        const int kk = k < NG ? k : 0;
+        gi[m][0] = T::gidx[3 * kk];
+        gi[m][1] = TW * N + T::gidx[3 * kk + 1];
+        gi[m][2] = 2 * TW * N + T::gidx[3 * kk + 2];
+    }
+    const int ax = lane / N, rr = lane - (lane / N) * N;
+    for (int ip = 0; ip < bra.n; ++ip) {
+        const double za = bra.zeta[ip], Ka = bra.K[ip];
+        const double Px = bra.Px[ip], Py = bra.Py[ip], Pz = bra.Pz[ip];
+        for (int iq = 0; iq < ket.n; ++iq) {
+            if (lane < 3 * N) {
+                const double zc = ket.zeta[iq], Kc = ket.K[iq];
+                const double Qx = ket.Px[iq], Qy = ket.Py[iq], Qz = ket.Pz[iq];
+                const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
+                const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
+                const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
+                double u[N], w[N];
+                rys_roots<N>(X, u, w);
+                double ur = u[0], wr = w[0];
+#pragma unroll
+                for (int r = 1; r < N; ++r)
+                    if (rr == r) { ur = u[r]; wr = w[r]; }
+                const double PA = (ax == 0 ? Px - bra.Ax : ax == 1 ? Py - bra.Ay : Pz - bra.Az);
+                const double QC = (ax == 0 ? Qx - ket.Ax : ax == 1 ? Qy - ket.Ay : Qz - ket.Az);
+                const double qp = QP[ax];
+                const double rz = ur * rho / za, re = ur * rho / zc;
+                const double b10 = (1.0 - rz) * 0.5 / za, b01 = (1.0 - re) * 0.5 / zc, b00 = ur * 0.5 * izs;
+                const double c00 = PA + rz * qp, d00 = QC - re * qp;
+                double t[W1][W2];
+                t[0][0] = ax == 2 ? wr * TWOPI5ON2 * Ka * Kc * sqrt(rho) : 1.0;
+#pragma unroll
+                for (int e = 0; e + 1 < W1; ++e)
+                    t[e + 1][0] = c00 * t[e][0] + (e ? e * b10 * t[e - 1][0] : 0.0);
+#pragma unroll
+                for (int f = 0; f + 1 < W2; ++f)
+#pragma unroll
+                    for (int e = 0; e < W1; ++e)
+                        t[e][f + 1] = d00 * t[e][f] + (f ? f * b01 * t[e][f - 1] : 0.0) +
+                                      (e ? e * b00 * t[e - 1][f] : 0.0);
+                double *dst = L.g + (ax * N + rr) * TW;
+#pragma unroll
+                for (int e = 0; e < W1; ++e)
+#pragma unroll
+                    for (int f = 0; f < W2; ++f) dst[e * W2 + f] = t[e][f];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < MG; ++m) {
+                double acc = 0.0;
+#pragma unroll
+                for (int r = 0; r < N; ++r)
+                    acc += L.g[gi[m][0] + r * TW] * L.g[gi[m][1] + r * TW] * L.g[gi[m][2] + r * TW];
+                Gr[m] += acc;
+            }
+            __syncthreads();
+        }
+    }
+    double *src = L.buf[0], *dst = L.buf[1];
+#pragma unroll
+    for (int m = 0; m < MG; ++m) {
+        const int k = lane + 64 * m;
+        if (k < NG) src[k] = Gr[m];
+    }
+    __syncthreads();
+    // ket-side transfer, rows e (T::NE of them), entries per level
+    const double CD[3] = {ket.ABx, ket.ABy, ket.ABz};
+    int nprev = T::NF;
+    for (int lev = 0; lev < T::NCD; ++lev) {
+        const int n = T::cd_n[lev], off = T::cd_off[lev];
+        for (int idx = lane; idx < T::NE * n; idx += 64) {
+            const int e = idx / n, t = idx - e * n;
+            dst[e * n + t] = src[e * nprev + T::cd_s1[off + t]] + CD[T::cd_ax[off + t]] * src[e * nprev + T::cd_s2[off + t]];
+        }
+        __syncthreads();
+        double *tmp = src; src = dst; dst = tmp;
+        nprev = n;
+    }
+    // bra-side transfer on columns rs
+    const double AB[3] = {bra.ABx, bra.ABy, bra.ABz};
+    for (int lev = 0; lev < T::NAB; ++lev) {
+        const int n = T::ab_n[lev], off = T::ab_off[lev];
+        for (int idx = lane; idx < n * NRS; idx += 64) {
+            const int t = idx / NRS, c = idx - t * NRS;
+            dst[t * NRS + c] = src[T::ab_s1[off + t] * NRS + c] + AB[T::ab_ax[off + t]] * src[T::ab_s2[off + t] * NRS + c];
+        }
+        __syncthreads();
+        double *tmp = src; src = dst; dst = tmp;
+    }
+    return src;
+}
+
+template <int LP, int LQ, int LR, int LS, bool MIXED>
+__global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
+{
+    using T = WaveTab<LP, LQ, LR, LS>;
+    constexpr int NP = T::NP, NQ = T::NQ, NR = T::NR, NS = T::NS;
+    constexpr int NPQ = NP * NQ, NRS = NR * NS, MJ = (NPQ + 63) / 64;
+    __shared__ WaveLDS<LP, LQ, LR, LS> L;
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x;
+    if (i >= a.n_bra) return;
+    const int sb = a.bra_start + i;
+    const int nbf = a.nbf;
+    const int tb = a.pr.tonto[sb];
+    const double mIb = a.pr.maxI[sb];
+    const int bp = a.pr.sh_p[sb], bq = a.pr.sh_q[sb];
+    const int atp = a.pr.atom_p[sb], atq = a.pr.atom_q[sb];
+    const int fp = a.pr.bf_p[sb], fq = a.pr.bf_q[sb];
+    const double Pmax_b = a.Pmax[tb];
+    const double P_max = a.scal[0], IP_max = a.scal[1];
+    const double *__restrict__ P = a.P;
+    const double *__restrict__ maxI = a.pr.maxI + a.ket_start;
+    const int jlim = a.same_bin ? i + 1 : a.n_ket;
+    int lo = 0, hi = jlim;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (mIb * maxI[mid] * P_max < a.cutoff) hi = mid;
+        else lo = mid + 1;
+    }
+    const int jend = lo;
+    for (int t = lane; t < NPQ; t += 64) L.Ppq[t] = P[(size_t)(fp + t / NQ) * nbf + fq + t % NQ];
+    double Jpq[MJ];
+#pragma unroll
+    for (int m = 0; m < MJ; ++m) Jpq[m] = 0.0;
+    unsigned long long nq = 0, npq = 0;
+    for (int j = 0; j < jend; ++j) {  // uniform over the work-group
+        const int sk = a.ket_start + j;
+        const int tk = a.pr.tonto[sk];
+        const double mIk = maxI[j];
+        const bool bhi = tb >= tk;
+        const double Iab = bhi ? mIb : mIk, Icd = bhi ? mIk : mIb;
+        if (Iab * IP_max < a.cutoff) continue;
+        const int kr = a.pr.sh_p[sk], ks = a.pr.sh_q[sk];
+        const double Pmax_k = a.Pmax[tk];
+        const double JP = bhi ? fmax(Pmax_b, Pmax_k) : fmax(Pmax_k, Pmax_b);
+        const double KP = fmax(fmax(a.Pmax[tonto_pair(bp, kr)], a.Pmax[tonto_pair(bp, ks)]),
+                               fmax(a.Pmax[tonto_pair(bq, kr)], a.Pmax[tonto_pair(bq, ks)]));
+        const bool skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
+        const bool skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
+        if (skipJ && skipK) continue;
+        double f = (bp == bq) ? 0.5 : 1.0;
+        if (kr == ks) f *= 0.5;
+        if (sb == sk) f *= 0.5;
+        const int akr = a.pr.atom_p[sk], aks = a.pr.atom_q[sk];
+        const bool any_same = atp == atq || akr == aks || atp == akr || atp == aks ||
+                              atq == akr || atq == aks;
+        const PairView bv = make_view(a.pr, a.pm, sb, !any_same);
+        const PairView kv = make_view(a.pr, a.pm, sk, !any_same);
+        const int fr = a.pr.bf_p[sk], fs = a.pr.bf_q[sk];
+        // stage the ket-dependent density blocks
+        for (int t = lane; t < NRS; t += 64) L.Prs[t] = P[(size_t)(fr + t / NS) * nbf + fs + t % NS];
+        for (int t = lane; t < NS * NQ; t += 64) L.Psq[t] = P[(size_t)(fs + t / NQ) * nbf + fq + t % NQ];
+        for (int t = lane; t < NR * NQ; t += 64) L.Prq[t] = P[(size_t)(fr + t / NQ) * nbf + fq + t % NQ];
+        for (int t = lane; t < NS * NP; t += 64) L.Psp[t] = P[(size_t)(fs + t / NP) * nbf + fp + t % NP];
+        for (int t = lane; t < NR * NP; t += 64) L.Prp[t] = P[(size_t)(fr + t / NP) * nbf + fp + t % NP];
+        const double *I = wave_eri<LP, LQ, LR, LS>(bv, kv, L);
+        ++nq;
+        npq += (unsigned long long)(bv.n * kv.n);
+        if (!skipJ) {
+            for (int t = lane; t < NRS; t += 64) {
+                double acc = 0.0;
+                for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + t] * L.Ppq[pq];
+                unsafeAtomicAdd(&a.X[(size_t)(fr + t / NS) * nbf + fs + t % NS], f * acc);
+            }
+#pragma unroll
+            for (int m = 0; m < MJ; ++m) {
+                const int pq = lane + 64 * m;
+                if (pq < NPQ) {
+                    double acc = 0.0;
+                    for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * L.Prs[rs];
+                    Jpq[m] += f * acc;
+                }
+            }
+        }
+        if (!skipK) {
+            constexpr int N1 = NP * NR, N2 = N1 + NP * NS, N3 = N2 + NQ * NR, N4 = N3 + NQ * NS;
+            for (int t = lane; t < N4; t += 64) {
+                double acc = 0.0;
+                size_t dst;
+                if (t < N1) {  // K(p,r) += I P(s,q)
+                    const int p = t / NR, r = t % NR;
+                    for (int q = 0; q < NQ; ++q)
+                        for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psq[s * NQ + q];
+                    dst = (size_t)(fp + p) * nbf + fr + r;
+                } else if (t < N2) {  // K(p,s) += I P(r,q)
+                    const int p = (t - N1) / NS, s = (t - N1) % NS;
+                    for (int q = 0; q < NQ; ++q)
+                        for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prq[r * NQ + q];
+                    dst = (size_t)(fp + p) * nbf + fs + s;
+                } else if (t < N3) {  // K(q,r) += I P(s,p)
+                    const int q = (t - N2) / NR, r = (t - N2) % NR;
+                    for (int p = 0; p < NP; ++p)
+                        for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psp[s * NP + p];
+                    dst = (size_t)(fq + q) * nbf + fr + r;
+                } else {  // K(q,s) += I P(r,p)
+                    const int q = (t - N3) / NS, s = (t - N3) % NS;
+                    for (int p = 0; p < NP; ++p)
+                        for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prp[r * NP + p];
+                    dst = (size_t)(fq + q) * nbf + fs + s;
+                }
+                unsafeAtomicAdd(&a.Y[dst], f * acc);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < MJ; ++m) {
+        const int pq = lane + 64 * m;
+        if (pq < NPQ && Jpq[m] != 0.0)
+            unsafeAtomicAdd(&a.X[(size_t)(fp + pq / NQ) * nbf + fq + pq % NQ], Jpq[m]);
+    }
+    if (lane == 0 && nq) {
+        atomicAdd(&a.counters[3 * a.counter_slot + 0], nq);
+        atomicAdd(&a.counters[3 * a.counter_slot + 1], npq);
+    }
+}
+
+template <int LP, int LQ>
+__global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims pm, int start, int n,
+                                                         double *maxI)
+{
+    using T = WaveTab<LP, LQ, LP, LQ>;
+    constexpr int NPQ = T::NP * T::NQ;
+    __shared__ WaveLDS<LP, LQ, LP, LQ> L;
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int s = start + i;
+    const PairView v = make_view(pr, pm, s, false);
+    const double *I = wave_eri<LP, LQ, LP, LQ>(v, v, L);
+    double m = 0.0;
+    for (int t = threadIdx.x; t < NPQ; t += 64) m = fmax(m, fabs(I[t * NPQ + t]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x == 0) maxI[s] = sqrt(m);
+}
+
+// Launchers defined in the per-class translation units (jk_class.hip)
+#define TGPU_DECLARE_JK(LP, LQ, LR, LS, H) \
+    void tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, bool mixed, hipStream_t s);
+TGPU_FOR_EACH_CLASS(TGPU_DECLARE_JK)
+#undef TGPU_DECLARE_JK
+#define TGPU_DECLARE_SCHWARZ(LP, LQ, H) \
+    void tgpu_launch_schwarz_##LP##LQ(const DevPairs &pr, const DevPrims &pm, int start, int n, double *maxI, hipStream_t s);
+TGPU_FOR_EACH_PAIRCLASS(TGPU_DECLARE_SCHWARZ)
+#undef TGPU_DECLARE_SCHWARZ

@@ -1,0 +1,50 @@
+"""GPU-resident RHF: one-electron matrices, Schwarz table and every J/K build on the
+device through libtonto_gpu's C ABI; the SCF algebra (DIIS, eigen-solve, density) in
+torch on the same device and stream.  Mirrors MOLECULE.SCF:usual_SCF
+(foofiles/molecule.scf.foo:1690-1727) with make_scf_Fock_mx (1562-1620) as the J/K call.
+"""
+import time
+
+import torch
+
+from .gpu import JKEngine
+from .scf import run_rhf
+
+
+class GPUSCF:
+    def __init__(self, mol, schwarz=1e-9, j_density=1e-9, k_density=1e-9, pair=1e-6,
+                 fp32_threshold=1e-6, device="cuda"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("GPUSCF needs a HIP device")
+        self.mol = mol
+        self.device = torch.device(device)
+        t0 = time.perf_counter()
+        self.engine = JKEngine(mol, schwarz=schwarz, j_density=j_density, k_density=k_density,
+                               pair=pair, fp32_threshold=fp32_threshold)
+        S, T, V = self.engine.one_electron()
+        self.S = torch.tensor(S, device=self.device)
+        self.h = torch.tensor(T + V, device=self.device)
+        self.max_I = self.engine.schwarz()
+        self.setup_s = time.perf_counter() - t0
+        n = mol.n_bf
+        self._J = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        self._K = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        self.build_log = []
+
+    def jk(self, P, mode="full"):
+        """One J/K build on the device; mode as in tonto_amd.scf.run_rhf."""
+        P = P.contiguous()
+        self.engine.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        self.engine.make_r_JK_device(P.data_ptr(), self._J.data_ptr(), self._K.data_ptr(),
+                                     mixed=mode.endswith("_mixed"))
+        return self._J.clone(), self._K.clone()
+
+    def run(self, precision="fp64", mixed_until=1e-4, incremental=False, log=None, **kw):
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        r = run_rhf(self.h, self.S, self.mol.n_electrons, self.mol.nuclear_repulsion(), self.jk,
+                    mixed_until=mixed_until if precision == "mixed" else None,
+                    incremental=incremental, log=log, **kw)
+        torch.cuda.synchronize(self.device)
+        r["wall"] = time.perf_counter() - t0
+        return r

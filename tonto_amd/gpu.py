@@ -1,0 +1,167 @@
+"""Python host binding of libtonto_gpu.so (the C ABI in include/tonto_gpu.h).
+
+This is the product path: it loads the in-tree HIP library and raises if it is
+missing or if no GPU is present; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtonto_gpu.so")
+
+TGPU_JK_MIXED = 1
+
+# exported symbols, in the order of include/tonto_gpu.h (plus the two extensions)
+EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_set_stream",
+           "tgpu_set_cutoffs", "tgpu_set_fp32_threshold", "tgpu_schwarz", "tgpu_make_r_JK",
+           "tgpu_make_r_JK_device", "tgpu_get_stats", "tgpu_one_electron",
+           "tgpu_get_class_counters", "tgpu_n_class_pairs", "tgpu_set_profiling",
+           "tgpu_get_class_times"]
+
+_lib = None
+
+
+class TGPUError(RuntimeError):
+    pass
+
+
+class _System(ctypes.Structure):
+    _fields_ = [("n_atom", ctypes.c_int), ("n_shell", ctypes.c_int),
+                ("atom_xyz", ctypes.POINTER(ctypes.c_double)),
+                ("atom_Z", ctypes.POINTER(ctypes.c_double)),
+                ("shell_l", ctypes.POINTER(ctypes.c_int)),
+                ("shell_atom", ctypes.POINTER(ctypes.c_int)),
+                ("shell_nprim", ctypes.POINTER(ctypes.c_int)),
+                ("exps", ctypes.POINTER(ctypes.c_double)),
+                ("coefs", ctypes.POINTER(ctypes.c_double))]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("quartets", ctypes.c_ulonglong), ("prim_quartets", ctypes.c_ulonglong),
+                ("fp32_quartets", ctypes.c_ulonglong), ("kernel_ms", ctypes.c_double)]
+
+
+def lib():
+    """Load libtonto_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TGPUError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+        L.tgpu_create.argtypes = [ctypes.POINTER(_System), ctypes.POINTER(vp)]
+        L.tgpu_destroy.argtypes = [vp]
+        L.tgpu_last_error.argtypes = [vp]
+        L.tgpu_last_error.restype = ctypes.c_char_p
+        L.tgpu_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
+        L.tgpu_set_stream.argtypes = [vp, vp]
+        L.tgpu_set_cutoffs.argtypes = [vp] + [ctypes.c_double] * 4
+        L.tgpu_set_fp32_threshold.argtypes = [vp, ctypes.c_double]
+        L.tgpu_schwarz.argtypes = [vp, vp]
+        L.tgpu_make_r_JK.argtypes = [vp, dp, dp, dp, ctypes.c_int]
+        L.tgpu_make_r_JK_device.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+        L.tgpu_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.tgpu_one_electron.argtypes = [vp, dp, dp, dp]
+        L.tgpu_get_class_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]
+        L.tgpu_n_class_pairs.restype = ctypes.c_int
+        L.tgpu_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.tgpu_get_class_times.argtypes = [vp, dp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class JKEngine:
+    """One molecule's context on the current HIP device (mirrors Tonto's
+    MOLECULE state consumed by make_r_JK_engine)."""
+
+    def __init__(self, mol, schwarz=1e-9, j_density=1e-9, k_density=1e-9, pair=1e-6,
+                 fp32_threshold=1e-6):
+        self.mol = mol
+        a = mol.shell_arrays()
+        self._keep = a
+        self._sys = _System(mol.n_atom, mol.n_shell, _dp(a["atom_xyz"]), _dp(a["atom_Z"]),
+                            _ip(a["l"]), _ip(a["atom"]), _ip(a["nprim"]), _dp(a["exps"]),
+                            _dp(a["coefs"]))
+        self._ctx = ctypes.c_void_p()
+        rc = lib().tgpu_create(ctypes.byref(self._sys), ctypes.byref(self._ctx))
+        self._check(rc)
+        self.n_bf = mol.n_bf
+        self.n_pairs = mol.n_shell * (mol.n_shell + 1) // 2
+        self._check(lib().tgpu_set_cutoffs(self._ctx, schwarz, j_density, k_density, pair))
+        self._check(lib().tgpu_set_fp32_threshold(self._ctx, fp32_threshold))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().tgpu_last_error(self._ctx).decode() if self._ctx else ""
+            raise TGPUError(f"libtonto_gpu error {rc}: {msg}")
+
+    def close(self):
+        if self._ctx:
+            lib().tgpu_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        self._check(lib().tgpu_set_stream(self._ctx, ctypes.c_void_p(stream_ptr)))
+
+    def schwarz(self):
+        out = np.zeros(self.n_pairs)
+        self._check(lib().tgpu_schwarz(self._ctx, out.ctypes.data))
+        return out
+
+    def make_r_JK(self, P, mixed=False):
+        """Host arrays in, host arrays out (synchronous)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n = self.n_bf
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        self._check(lib().tgpu_make_r_JK(self._ctx, _dp(P), _dp(J), _dp(K),
+                                         TGPU_JK_MIXED if mixed else 0))
+        return J, K
+
+    def make_r_JK_device(self, P_ptr, J_ptr, K_ptr, mixed=False):
+        """Device pointers (e.g. torch tensor .data_ptr()), asynchronous on the stream."""
+        self._check(lib().tgpu_make_r_JK_device(self._ctx, ctypes.c_void_p(P_ptr),
+                                                ctypes.c_void_p(J_ptr), ctypes.c_void_p(K_ptr),
+                                                TGPU_JK_MIXED if mixed else 0))
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().tgpu_get_stats(self._ctx, ctypes.byref(s)))
+        return dict(quartets=s.quartets, prim_quartets=s.prim_quartets,
+                    fp32_quartets=s.fp32_quartets, kernel_ms=s.kernel_ms)
+
+    def class_counters(self):
+        n = lib().tgpu_n_class_pairs()
+        buf = (ctypes.c_ulonglong * (3 * n))()
+        self._check(lib().tgpu_get_class_counters(self._ctx, buf, 3 * n))
+        return np.array(buf[:], dtype=np.uint64).reshape(n, 3)
+
+    def set_profiling(self, on=True):
+        self._check(lib().tgpu_set_profiling(self._ctx, int(on)))
+
+    def class_times(self):
+        n = lib().tgpu_n_class_pairs()
+        out = np.zeros(n)
+        self._check(lib().tgpu_get_class_times(self._ctx, _dp(out), n))
+        return out
+
+    def one_electron(self):
+        n = self.n_bf
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        self._check(lib().tgpu_one_electron(self._ctx, _dp(S), _dp(T), _dp(V)))
+        return S, T, V

@@ -10,8 +10,7 @@ E = 1/2 tr P(F+h) + V_NN (`molecule.scf.foo:4398-4433`) -> converged when |dE| a
 DIIS error are both below their tolerances (`scf_data.foo:2283-2313`).
 
 Matrices are torch tensors on any device; the J/K provider decides where the Fock
-build runs.  The provider signature is jk(P, mode) -> (J, K) where mode is "full" or
-"delta" (a build from a density difference, used by incremental builds).
+build runs (see run_rhf for its signature).
 """
 import time
 
@@ -36,34 +35,39 @@ def core_guess(h, X, n_occ):
 
 def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e-6,
             max_iterations=100, min_iterations=1, diis_keep=8, diis_start=3,
-            level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3, incremental=None,
-            log=None):
+            level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3,
+            mixed_until=None, incremental=False, rebuild_every=8, log=None):
     """Run RHF to convergence.
 
-    incremental: None, or dict(start=k, rebuild_every=m, final_full=True): from
-      iteration k on, the 2e Fock part is updated with a build on P - P_prev
-      ("delta" mode); every m-th iteration and once converged a full build is done,
-      and convergence is only accepted on a full build.
+    jk(P, mode) returns (J, K); mode is "full" or "delta" (a build from a density
+    difference), with the suffix "_mixed" while low precision is allowed.
+    mixed_until: None for FP64 builds throughout; otherwise builds are "_mixed" until
+      |dE| < mixed_until, then a full FP64 build is forced and FP64 is kept to the end.
+    incremental: update the 2e Fock part with builds on P - P_prev; every rebuild_every-th
+      iteration, at the precision switch and once converged a full build is done, and
+      convergence is only accepted on a full FP64 build.
     """
     dt = h.dtype
     n_occ = n_electrons // 2
     X = _sym_orth(S)
     P = core_guess(h, X, n_occ) if P0 is None else P0.clone()
+    low = mixed_until is not None
     t0 = time.perf_counter()
-    J, K = jk(P, "full")
+    J, K = jk(P, "full_mixed" if low else "full")
     G = J - 0.5 * K
     F = h + G
     E = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
     history = []
     diis_F, diis_E = [], []
-    n_builds = {"full": 1, "delta": 0}
+    n_builds = {"full": 0, "delta": 0, "full_mixed": 0, "delta_mixed": 0}
+    n_builds["full_mixed" if low else "full"] += 1
     P_build = P
-    last_full = True
+    last_full_fp64 = not low
+    force_full = False
     converged = False
     it = 0
     while True:
         it += 1
-        # DIIS error and extrapolation
         err = X.T @ (F @ P @ S - S @ P @ F) @ X
         err_max = err.abs().max().item()
         diis_F.append(F)
@@ -88,10 +92,10 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
                 F_use = sum(ci * Fi for ci, Fi in zip(c, diis_F))
             except RuntimeError:
                 F_use = F
-        # diagonalise (level shift on virtual space in early iterations)
         Fp = X.T @ F_use @ X
         if level_shift and it < level_shift_iters:
-            Cocc_p = torch.linalg.pinv(X) @ (P / 2.0) @ torch.linalg.pinv(X).T
+            Xi = torch.linalg.pinv(X)
+            Cocc_p = Xi @ (P / 2.0) @ Xi.T
             Fp = Fp + level_shift * (torch.eye(Fp.shape[0], dtype=dt, device=h.device) - Cocc_p)
         eps, Cp = torch.linalg.eigh(Fp)
         C = X @ Cp
@@ -99,42 +103,44 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         P_new = 2.0 * Co @ Co.T
         if damp and it < damp_iters:
             P_new = damp * P + (1.0 - damp) * P_new
-        # Fock build: full or incremental
-        full = True
-        if incremental and it >= incremental.get("start", 2):
-            every = incremental.get("rebuild_every", 8)
-            full = (it % every == 0)
+        full = (not incremental) or force_full or it % rebuild_every == 0
+        mode = ("full" if full else "delta") + ("_mixed" if low else "")
         if full:
-            J, K = jk(P_new, "full")
+            J, K = jk(P_new, mode)
             G = J - 0.5 * K
-            n_builds["full"] += 1
         else:
-            dJ, dK = jk(P_new - P_build, "delta")
+            dJ, dK = jk(P_new - P_build, mode)
             G = G + dJ - 0.5 * dK
-            n_builds["delta"] += 1
+        n_builds[mode] += 1
+        force_full = False
+        last_full_fp64 = full and not low
         P_build = P_new
-        last_full = full
         P = P_new
         F = h + G
         E_new = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
         dE = E_new - E
         E = E_new
-        history.append(dict(iteration=it, energy=E, dE=dE, diis_error=err_max, full=full))
+        history.append(dict(iteration=it, energy=E, dE=dE, diis_error=err_max, mode=mode))
         if log:
-            log(f"iter {it:3d} E={E:.12f} dE={dE:+.3e} err={err_max:.3e} {'full' if full else 'delta'}")
+            log(f"iter {it:3d} E={E:.12f} dE={dE:+.3e} err={err_max:.3e} {mode}")
         done = abs(dE) < convergence and err_max < diis_tol
-        if done and incremental and not last_full:
-            # win back the incremental error: one full build on the current density
+        if low:
+            if abs(dE) < mixed_until or done:
+                low = False
+                force_full = True
+            done = False
+        elif done and not last_full_fp64:
+            # win back the incremental error: a full FP64 build on the current density
             J, K = jk(P, "full")
-            G = J - 0.5 * K
             n_builds["full"] += 1
+            G = J - 0.5 * K
             F = h + G
             E_full = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
             history.append(dict(iteration=it, energy=E_full, dE=E_full - E, diis_error=err_max,
-                                full=True))
+                                mode="full"))
             done = abs(E_full - E) < convergence
             E = E_full
-            last_full = True
+            last_full_fp64 = True
         if (done and it >= min_iterations) or it >= max_iterations:
             converged = done
             break
