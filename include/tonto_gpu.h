@@ -40,6 +40,8 @@ enum {
 
 /* flags for tgpu_make_r_JK* */
 #define TGPU_JK_MIXED 1    /* quartets with Schwarz bound below the FP32 threshold run in FP32 */
+#define TGPU_JK_GRAPH 2    /* device entry only: capture the build once into a hipGraph and
+                              replay it while P/J/K pointers, flags and class forms are unchanged */
 
 typedef struct tgpu_ctx tgpu_ctx;
 
@@ -75,10 +77,19 @@ typedef struct {
  * primitive-pair data with and without the 1e-6 pair skip (shell1pair.foo:100-156,
  * shell1quartet.foo:131-420) and uploads them. */
 int tgpu_create(const tgpu_system *sys, tgpu_ctx **ctx);
+
+/* A batch of n_mol independent molecules whose J/K builds run in the same launches
+ * (each molecule's build is Tonto's make_r_JK_engine on that molecule alone).  Matrices
+ * (P, J, K, S, T, V) are then packed molecule after molecule, n_bf(m)^2 doubles each;
+ * max_I is packed in molecule order.  No Tonto counterpart (Tonto builds one molecule). */
+int tgpu_create_batch(const tgpu_system *systems, int n_mol, tgpu_ctx **ctx);
+int tgpu_batch_layout(const tgpu_ctx *ctx, int *n_mol, long long *n_elements,
+                      long long *mat_off /* [n_mol] */, int *n_bf /* [n_mol] */,
+                      long long *pair_off /* [n_mol+1] */);
 int tgpu_destroy(tgpu_ctx *ctx);
 const char *tgpu_last_error(const tgpu_ctx *ctx);
 
-/* Basis dimensions: n_bf, n_shell_pairs. */
+/* Basis dimensions of the (first) molecule: n_bf, n_shell_pairs. */
 int tgpu_dims(const tgpu_ctx *ctx, int *n_bf, long long *n_shell_pairs);
 
 /* Issue all further work on this hipStream_t (NULL = the null stream). */
@@ -126,6 +137,15 @@ int tgpu_get_class_counters(tgpu_ctx *ctx, unsigned long long *out, int n);
  * profiling is off by default.  No Tonto counterpart: measurement support. */
 int tgpu_set_profiling(tgpu_ctx *ctx, int on);
 int tgpu_get_class_times(tgpu_ctx *ctx, double *ms, int n);
+
+/* Per-class-pair kernel form: 0 = one lane per quartet (classes with <= 81 integrals),
+ * 1 = one wavefront per quartet (every class).  tgpu_autotune times both forms on a build
+ * of the given device density and keeps the faster one per class pair; ms_out (optional,
+ * 2 doubles per class pair) receives the two times (-1 where form 0 does not exist).
+ * No Tonto counterpart: kernel selection. */
+int tgpu_set_variant(tgpu_ctx *ctx, int class_pair, int variant);
+int tgpu_get_variant(const tgpu_ctx *ctx, int class_pair);
+int tgpu_autotune(tgpu_ctx *ctx, const double *P_device, int flags, double *ms_out);
 
 /* One-electron matrices S, T, V (n_bf*n_bf doubles, host arrays), the core-Hamiltonian
  * inputs of the SCF (MOLECULE.INTS: make_overlap_mx, make_kinetic_energy_mx,

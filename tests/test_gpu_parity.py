@@ -150,3 +150,90 @@ def test_gpu_scf_reaches_g09(torch_cuda):
     from tonto_amd.driver import GPUSCF
     r = GPUSCF(tonto_water("cc-pVDZ")).run(convergence=1e-10, diis_tol=1e-7)
     assert r.converged and abs(r.energy - (-76.0232731206)) < 1e-9
+
+
+def _uneven_batch(n_mol=6, basis="6-31G(d,p)", seed=7):
+    from tonto_amd.geometry import water_cluster
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, 6, size=n_mol)
+    return [water_cluster(int(k), basis, seed=int(seed + i)) for i, k in enumerate(sizes)]
+
+
+def test_batch_matches_oracle_per_molecule(torch_cuda, oracle_mod):
+    mols = _uneven_batch()
+    Ps = [_rand_density(m.n_bf, 10 + i, 0.3) for i, m in enumerate(mols)]
+    e = _engine(mols)
+    mi = e.schwarz()
+    J, K = e.make_r_JK(Ps)
+    tot = 0
+    for i, m in enumerate(mols):
+        o = oracle_mod.OracleSystem(m)
+        mo = o.schwarz()
+        seg = mi[e.pair_off[i]:e.pair_off[i + 1]]
+        assert np.max(np.abs(seg - mo) / np.maximum(mo, 1e-300)) < 1e-13
+        Jo, Ko, so = o.make_r_JK(Ps[i], max_I=seg, return_stats=True)
+        dF = np.max(np.abs((J[i] - 0.5 * K[i]) - (Jo - 0.5 * Ko)))
+        assert dF <= 1e-10, (i, dF)
+        tot += so["quartets"]
+    assert e.stats()["quartets"] == tot
+
+
+def test_batch_equals_single_builds(torch_cuda):
+    mols = _uneven_batch(4, "6-31G(d)", 3)
+    Ps = [_rand_density(m.n_bf, 20 + i, 0.3) for i, m in enumerate(mols)]
+    e = _engine(mols)
+    J, K = e.make_r_JK(Ps)
+    S, T, V = e.one_electron()
+    for i, m in enumerate(mols):
+        s = _engine(m)
+        J1, K1 = s.make_r_JK(Ps[i])
+        S1, T1, V1 = s.one_electron()
+        assert np.max(np.abs(J[i] - J1)) < 1e-12 and np.max(np.abs(K[i] - K1)) < 1e-12
+        assert np.max(np.abs(S[i] - S1)) < 1e-14 and np.max(np.abs(V[i] - V1)) < 1e-12
+
+
+def test_graph_replay_matches_eager(torch_cuda):
+    """Trap check: the captured build is replayed several times with eager work and
+    synchronisations in between, and every replay is compared with the eager build."""
+    import torch
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(3, "6-31G(d)")
+    e = _engine(mol)
+    P = torch.tensor(_rand_density(mol.n_bf, 6), device="cuda")
+    J0, K0 = (torch.empty_like(P) for _ in range(2))
+    J1, K1 = (torch.empty_like(P) for _ in range(2))
+    e.set_stream(torch.cuda.current_stream().cuda_stream)
+    e.make_r_JK_device(P.data_ptr(), J0.data_ptr(), K0.data_ptr())
+    torch.cuda.synchronize()
+    q0 = e.stats()["quartets"]
+    for rep in range(4):
+        J1.fill_(float("nan"))
+        K1.fill_(float("nan"))
+        e.make_r_JK_device(P.data_ptr(), J1.data_ptr(), K1.data_ptr(), graph=True)
+        torch.cuda.synchronize()
+        assert torch.max(torch.abs(J1 - J0)).item() < 1e-12
+        assert torch.max(torch.abs(K1 - K0)).item() < 1e-12
+        assert e.stats()["quartets"] == q0
+        # eager work on the same context between replays
+        e.make_r_JK_device(P.data_ptr(), J0.data_ptr(), K0.data_ptr())
+        torch.cuda.synchronize()
+
+
+def test_autotune_keeps_results(torch_cuda):
+    import torch
+    from tonto_amd.geometry import water_cluster
+    g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))
+    mol = water_cluster(8, "6-31G(d)", seed=int(g["seed"]))
+    e = _engine(mol, schwarz=1e-12)
+    J0, K0 = e.make_r_JK(g["P"])
+    P = torch.tensor(g["P"], device="cuda")
+    ms, var = e.autotune(P.data_ptr())
+    print("autotune variants", var)
+    assert all(v in (0, 1) for v in var)
+    J1, K1 = e.make_r_JK(g["P"])
+    assert np.max(np.abs(J1 - J0)) < 1e-12 and np.max(np.abs(K1 - K0)) < 1e-12
+    # every light class pair also through the wavefront form
+    for k in range(len(var)):
+        e.set_variant(k, 1)
+    J2, K2 = e.make_r_JK(g["P"])
+    assert np.max(np.abs(J2 - J0)) < 1e-12 and np.max(np.abs(K2 - K0)) < 1e-12

@@ -15,9 +15,10 @@ bra side (`shell1quartet.foo:8310-11134`), writing (pq|rs) as double in the orde
 R is the arithmetic type of the recurrences, accumulation and transfer (double or
 float); Rys roots and weights and the primitive prefactors are always FP64.
 
-Classes with more than HEAVY_NI integrals (or (e0|f0) entries) instead get
-`WaveTab<LP, LQ, LR, LS>`: index tables driving the wavefront-cooperative kernel
-(one 64-lane wave per quartet; 2-D tables and HRR levels staged in LDS).
+Every class also gets `WaveTab<LP, LQ, LR, LS>`: index tables driving the
+wavefront-cooperative kernel (one 64-lane wave per quartet; 2-D tables and HRR levels
+staged in LDS).  Classes with more than HEAVY_NI integrals (or (e0|f0) entries) get only
+that form.
 
 Output: gen/eri_<LP><LQ><LR><LS>.inc per class and gen/classes.inc, next to this file.
 """
@@ -294,10 +295,9 @@ def main():
     os.makedirs(gdir, exist_ok=True)
     pcs, qcs = classes(LMAX)
     for (LP, LQ, LR, LS) in qcs:
-        parts = ["/* Generated by gen_eri.py -- do not edit. */"]
-        if is_heavy(LP, LQ, LR, LS):
-            parts.append(gen_wave(LP, LQ, LR, LS))
-        else:
+        parts = ["/* Generated by gen_eri.py -- do not edit. */",
+                 gen_wave(LP, LQ, LR, LS)]
+        if not is_heavy(LP, LQ, LR, LS):
             for R in ("double", "float"):
                 parts.append(gen_class(LP, LQ, LR, LS, R))
         with open(os.path.join(gdir, f"eri_{LP}{LQ}{LR}{LS}.inc"), "w") as f:

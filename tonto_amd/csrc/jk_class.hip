@@ -14,16 +14,20 @@
 #define TGPU_LAUNCH_NAME_X(a, b, c, d) TGPU_CAT5(tgpu_launch_jk_, a, b, c, d)
 #define TGPU_SCHWARZ_NAME_X(a, b) TGPU_CAT3(tgpu_launch_schwarz_, a, b)
 
-void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, bool mixed, hipStream_t s)
+void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int variant, bool mixed,
+                                                          hipStream_t s)
 {
-#if TGPU_HEAVY
-    (void)mixed;  // heavy classes always run in FP64
-    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false><<<a.n_bra, 64, 0, s>>>(a);
-#else
-    const int blocks = (a.n_bra + 3) / 4;
-    if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true><<<blocks, 256, 0, s>>>(a);
-    else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false><<<blocks, 256, 0, s>>>(a);
+#if !TGPU_HEAVY
+    if (variant == 0) {
+        const int blocks = (a.n_bra + 3) / 4;
+        if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true><<<blocks, 256, 0, s>>>(a);
+        else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false><<<blocks, 256, 0, s>>>(a);
+        return;
+    }
 #endif
+    (void)variant;
+    (void)mixed;  // the wavefront form runs in FP64
+    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<a.n_bra, 64, 0, s>>>(a);
 }
 
 #if TGPU_LP == TGPU_LR && TGPU_LQ == TGPU_LS

@@ -79,7 +79,7 @@ template <int LP, int LQ, int LR, int LS, typename R> struct EriBody;
 template <int LP, int LQ, int LR, int LS> struct WaveTab;
 
 struct DevPairs {
-    const int *tonto, *sh_p, *sh_q, *bf_p, *bf_q, *atom_p, *atom_q;
+    const int *tonto, *sh_p, *sh_q, *bf_p, *bf_q, *atom_p, *atom_q, *mol;
     const int *off_full, *n_full, *off_skip, *n_skip;
     const double *A, *AB, *maxI;
 };
@@ -112,18 +112,108 @@ __device__ __forceinline__ long long tonto_pair(int x, int y)
     return x >= y ? (long long)x * (x + 1) / 2 + y : (long long)y * (y + 1) / 2 + x;
 }
 
+// Per-molecule layout of a batch: molecule m's pairs of class c occupy
+// [seg_start[m*NCLS+c], +seg_count) of the sorted pair arrays; its matrices start at
+// element mat_off[m] with leading dimension nbf[m]; its Tonto pairs start at pair_off[m]
+// in the P_max array; scal[2m], scal[2m+1] are its P_max and IP_max.
+struct DevMols {
+    const int *seg_start, *seg_count, *nbf;
+    const long long *mat_off, *pair_off;
+};
+
 struct JKArgs {
     DevPairs pr;
     DevPrims pm;
-    int bra_start, n_bra, ket_start, n_ket, same_bin, nbf, counter_slot;
+    DevMols ml;
+    int bra_start, n_bra, bra_cls, ket_cls, same_bin, counter_slot;
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
     unsigned long long *counters;
 };
 
+// Everything a wavefront needs about its bra pair, and the Schwarz frontier of its kets.
+struct BraInfo {
+    int sb, tb, bp, bq, atp, atq, fp, fq, ld, kb, jend;
+    double mIb, Pmax_b, IP_max;
+    const double *P, *Pmax, *maxI;
+    double *X, *Y;
+};
+
+__device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
+{
+    b.sb = a.bra_start + i;
+    const int m = a.pr.mol[b.sb];
+    b.kb = a.ml.seg_start[m * TGPU_NPAIRCLASS + a.ket_cls];
+    const int kn = a.ml.seg_count[m * TGPU_NPAIRCLASS + a.ket_cls];
+    const int jlim = a.same_bin ? b.sb - a.ml.seg_start[m * TGPU_NPAIRCLASS + a.bra_cls] + 1 : kn;
+    const long long mo = a.ml.mat_off[m];
+    b.ld = a.ml.nbf[m];
+    b.P = a.P + mo;
+    b.X = a.X + mo;
+    b.Y = a.Y + mo;
+    b.Pmax = a.Pmax + a.ml.pair_off[m];
+    const double P_max = a.scal[2 * m];
+    b.IP_max = a.scal[2 * m + 1];
+    b.tb = a.pr.tonto[b.sb];
+    b.mIb = a.pr.maxI[b.sb];
+    b.bp = a.pr.sh_p[b.sb];
+    b.bq = a.pr.sh_q[b.sb];
+    b.atp = a.pr.atom_p[b.sb];
+    b.atq = a.pr.atom_q[b.sb];
+    b.fp = a.pr.bf_p[b.sb];
+    b.fq = a.pr.bf_q[b.sb];
+    b.Pmax_b = b.Pmax[b.tb];
+    b.maxI = a.pr.maxI + b.kb;
+    // kets are sorted by max_I descending, so those passing
+    // max_I(ab)*max_I(cd)*P_max >= cutoff (molecule.fock.foo:1430) form a prefix
+    int lo = 0, hi = jlim;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (b.mIb * b.maxI[mid] * P_max < a.cutoff) hi = mid;
+        else lo = mid + 1;
+    }
+    b.jend = lo;
+}
+
+struct KetTest {
+    int sk;
+    bool skipJ, skipK, skip_lists;
+    double f;
+};
+
+// Tonto's remaining per-quartet tests (molecule.fock.foo:1412, 1432-1472 and
+// Schwarz_test 3497-3536), evaluated in Tonto's operand order.  Returns false if the
+// quartet contributes to neither J nor K.
+__device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int j, KetTest &t)
+{
+    t.sk = b.kb + j;
+    const int tk = a.pr.tonto[t.sk];
+    const double mIk = b.maxI[j];
+    const bool bhi = b.tb >= tk;  // Tonto's (ab| is the pair with the larger index
+    const double Iab = bhi ? b.mIb : mIk, Icd = bhi ? mIk : b.mIb;
+    if (Iab * b.IP_max < a.cutoff) return false;
+    const int kr = a.pr.sh_p[t.sk], ks = a.pr.sh_q[t.sk];
+    const double Pmax_k = b.Pmax[tk];
+    const double JP = bhi ? fmax(b.Pmax_b, Pmax_k) : fmax(Pmax_k, b.Pmax_b);
+    const double KP = fmax(fmax(b.Pmax[tonto_pair(b.bp, kr)], b.Pmax[tonto_pair(b.bp, ks)]),
+                           fmax(b.Pmax[tonto_pair(b.bq, kr)], b.Pmax[tonto_pair(b.bq, ks)]));
+    t.skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
+    t.skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
+    if (t.skipJ && t.skipK) return false;
+    double f = (b.bp == b.bq) ? 0.5 : 1.0;
+    if (kr == ks) f *= 0.5;
+    if (b.sb == t.sk) f *= 0.5;
+    t.f = f;
+    const int akr = a.pr.atom_p[t.sk], aks = a.pr.atom_q[t.sk];
+    const bool any_same = b.atp == b.atq || akr == aks || b.atp == akr || b.atp == aks ||
+                          b.atq == akr || b.atq == aks;
+    t.skip_lists = !any_same;
+    return true;
+}
+
 // ----------------------------------------------------------------------------------
-// The J/K kernel for one class pair (LP LQ | LR LS)
+// The J/K kernel for one light class pair (LP LQ | LR LS): one lane per quartet
 // ----------------------------------------------------------------------------------
 template <int LP, int LQ, int LR, int LS, bool MIXED>
 __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
@@ -133,75 +223,35 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= a.n_bra) return;  // wave-uniform
-    const int sb = a.bra_start + i;
-    const int nbf = a.nbf;
-    const int tb = a.pr.tonto[sb];
-    const double mIb = a.pr.maxI[sb];
-    const int bp = a.pr.sh_p[sb], bq = a.pr.sh_q[sb];
-    const int atp = a.pr.atom_p[sb], atq = a.pr.atom_q[sb];
-    const int fp = a.pr.bf_p[sb], fq = a.pr.bf_q[sb];
-    const double Pmax_b = a.Pmax[tb];
-    const double P_max = a.scal[0], IP_max = a.scal[1];
-    const double *__restrict__ P = a.P;
-    const double *__restrict__ maxI = a.pr.maxI + a.ket_start;
-
-    // Schwarz frontier: kets are sorted by max_I descending, so the kets passing
-    // max_I(ab)*max_I(cd)*P_max >= cutoff (molecule.fock.foo:1430) form a prefix.
-    const int jlim = a.same_bin ? i + 1 : a.n_ket;
-    int lo = 0, hi = jlim;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (mIb * maxI[mid] * P_max < a.cutoff) hi = mid;
-        else lo = mid + 1;
-    }
-    const int jend = lo;
-    int jsplit = jend;
+    BraInfo b;
+    load_bra(a, i, b);
+    const int ld = b.ld, fp = b.fp, fq = b.fq;
+    const double *__restrict__ P = b.P;
+    int jsplit = b.jend;
     if (MIXED) {
-        lo = 0;
-        hi = jend;
+        int lo = 0, hi = b.jend;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (mIb * maxI[mid] < a.fp32_thr) hi = mid;
+            if (b.mIb * b.maxI[mid] < a.fp32_thr) hi = mid;
             else lo = mid + 1;
         }
         jsplit = lo;
     }
-
     double Ppq[NPQ], Jpq[NPQ];
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            Ppq[p * NQ + q] = P[(size_t)(fp + p) * nbf + fq + q];
+            Ppq[p * NQ + q] = P[(size_t)(fp + p) * ld + fq + q];
             Jpq[p * NQ + q] = 0.0;
         }
     unsigned long long nq = 0, npq = 0, n32 = 0;
-
-    for (int j0 = 0; j0 < jend; j0 += 64) {
+    for (int j0 = 0; j0 < b.jend; j0 += 64) {
         const int j = j0 + lane;
-        if (j >= jend) continue;
-        const int sk = a.ket_start + j;
-        const int tk = a.pr.tonto[sk];
-        const double mIk = maxI[j];
-        const bool bhi = tb >= tk;  // Tonto's (ab| is the pair with the larger index
-        const double Iab = bhi ? mIb : mIk, Icd = bhi ? mIk : mIb;
-        if (Iab * IP_max < a.cutoff) continue;  // molecule.fock.foo:1412
-        const int kr = a.pr.sh_p[sk], ks = a.pr.sh_q[sk];
-        const double Pmax_k = a.Pmax[tk];
-        const double JP = bhi ? fmax(Pmax_b, Pmax_k) : fmax(Pmax_k, Pmax_b);
-        const double KP = fmax(fmax(a.Pmax[tonto_pair(bp, kr)], a.Pmax[tonto_pair(bp, ks)]),
-                               fmax(a.Pmax[tonto_pair(bq, kr)], a.Pmax[tonto_pair(bq, ks)]));
-        const bool skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
-        const bool skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
-        if (skipJ && skipK) continue;
-        double f = (bp == bq) ? 0.5 : 1.0;
-        if (kr == ks) f *= 0.5;
-        if (sb == sk) f *= 0.5;
-        const int akr = a.pr.atom_p[sk], aks = a.pr.atom_q[sk];
-        const bool any_same = atp == atq || akr == aks || atp == akr || atp == aks ||
-                              atq == akr || atq == aks;
-        const PairView bv = make_view(a.pr, a.pm, sb, !any_same);
-        const PairView kv = make_view(a.pr, a.pm, sk, !any_same);
+        KetTest t;
+        if (j >= b.jend || !test_ket(a, b, j, t)) continue;
+        const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
+        const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         double I[NI];
         if (!MIXED || j < jsplit) {
             EriBody<LP, LQ, LR, LS, double>::run(bv, kv, I);
@@ -211,19 +261,20 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
         }
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
-        const int fr = a.pr.bf_p[sk], fs = a.pr.bf_q[sk];
-        if (!skipJ) {
+        const double f = t.f;
+        const int fr = a.pr.bf_p[t.sk], fs = a.pr.bf_q[t.sk];
+        if (!t.skipJ) {
             double Prs[NRS];
 #pragma unroll
             for (int r = 0; r < NR; ++r)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Prs[r * NS + s] = P[(size_t)(fr + r) * nbf + fs + s];
+                for (int s = 0; s < NS; ++s) Prs[r * NS + s] = P[(size_t)(fr + r) * ld + fs + s];
 #pragma unroll
             for (int rs = 0; rs < NRS; ++rs) {
                 double acc = 0.0;
 #pragma unroll
                 for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + rs] * Ppq[pq];
-                unsafeAtomicAdd(&a.X[(size_t)(fr + rs / NS) * nbf + fs + rs % NS], f * acc);
+                unsafeAtomicAdd(&b.X[(size_t)(fr + rs / NS) * ld + fs + rs % NS], f * acc);
             }
 #pragma unroll
             for (int pq = 0; pq < NPQ; ++pq) {
@@ -233,23 +284,23 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
                 Jpq[pq] += f * acc;
             }
         }
-        if (!skipK) {
+        if (!t.skipK) {
             // K(p,r) += I P(s,q); K(p,s) += I P(r,q); K(q,r) += I P(s,p); K(q,s) += I P(r,p)
             // (shell1quartet.foo:11508-11521)
             double Psq[NS * NQ], Prq[NR * NQ], Psp[NS * NP], Prp[NR * NP];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fs + s) * nbf + fq + q];
+                for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fs + s) * ld + fq + q];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fr + r) * nbf + fq + q];
+                for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fr + r) * ld + fq + q];
             }
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fs + s) * nbf + fp + p];
+                for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fs + s) * ld + fp + p];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fr + r) * nbf + fp + p];
+                for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fr + r) * ld + fp + p];
             }
 #pragma unroll
             for (int p = 0; p < NP; ++p)
@@ -261,7 +312,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Psq[s * NQ + q];
-                    unsafeAtomicAdd(&a.Y[(size_t)(fp + p) * nbf + fr + r], f * acc);
+                    unsafeAtomicAdd(&b.Y[(size_t)(fp + p) * ld + fr + r], f * acc);
                 }
 #pragma unroll
             for (int p = 0; p < NP; ++p)
@@ -273,7 +324,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Prq[r * NQ + q];
-                    unsafeAtomicAdd(&a.Y[(size_t)(fp + p) * nbf + fs + s], f * acc);
+                    unsafeAtomicAdd(&b.Y[(size_t)(fp + p) * ld + fs + s], f * acc);
                 }
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -285,7 +336,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Psp[s * NP + p];
-                    unsafeAtomicAdd(&a.Y[(size_t)(fq + q) * nbf + fr + r], f * acc);
+                    unsafeAtomicAdd(&b.Y[(size_t)(fq + q) * ld + fr + r], f * acc);
                 }
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -297,7 +348,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Prp[r * NP + p];
-                    unsafeAtomicAdd(&a.Y[(size_t)(fq + q) * nbf + fs + s], f * acc);
+                    unsafeAtomicAdd(&b.Y[(size_t)(fq + q) * ld + fs + s], f * acc);
                 }
         }
     }
@@ -308,7 +359,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if (lane == 0 && v != 0.0)
-            unsafeAtomicAdd(&a.X[(size_t)(fp + pq / NQ) * nbf + fq + pq % NQ], v);
+            unsafeAtomicAdd(&b.X[(size_t)(fp + pq / NQ) * ld + fq + pq % NQ], v);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -468,7 +519,7 @@ __device__ const double *wave_eri(const PairView &bra, const PairView &ket, Wave
     return src;
 }
 
-template <int LP, int LQ, int LR, int LS, bool MIXED>
+template <int LP, int LQ, int LR, int LS>
 __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
@@ -476,70 +527,37 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
     constexpr int NPQ = NP * NQ, NRS = NR * NS, MJ = (NPQ + 63) / 64;
     __shared__ WaveLDS<LP, LQ, LR, LS> L;
     const int lane = threadIdx.x;
-    const int i = blockIdx.x;
-    if (i >= a.n_bra) return;
-    const int sb = a.bra_start + i;
-    const int nbf = a.nbf;
-    const int tb = a.pr.tonto[sb];
-    const double mIb = a.pr.maxI[sb];
-    const int bp = a.pr.sh_p[sb], bq = a.pr.sh_q[sb];
-    const int atp = a.pr.atom_p[sb], atq = a.pr.atom_q[sb];
-    const int fp = a.pr.bf_p[sb], fq = a.pr.bf_q[sb];
-    const double Pmax_b = a.Pmax[tb];
-    const double P_max = a.scal[0], IP_max = a.scal[1];
-    const double *__restrict__ P = a.P;
-    const double *__restrict__ maxI = a.pr.maxI + a.ket_start;
-    const int jlim = a.same_bin ? i + 1 : a.n_ket;
-    int lo = 0, hi = jlim;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (mIb * maxI[mid] * P_max < a.cutoff) hi = mid;
-        else lo = mid + 1;
-    }
-    const int jend = lo;
-    for (int t = lane; t < NPQ; t += 64) L.Ppq[t] = P[(size_t)(fp + t / NQ) * nbf + fq + t % NQ];
+    if ((int)blockIdx.x >= a.n_bra) return;
+    BraInfo b;
+    load_bra(a, blockIdx.x, b);
+    const int ld = b.ld, fp = b.fp, fq = b.fq;
+    const double *__restrict__ P = b.P;
+    for (int t = lane; t < NPQ; t += 64) L.Ppq[t] = P[(size_t)(fp + t / NQ) * ld + fq + t % NQ];
     double Jpq[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; ++m) Jpq[m] = 0.0;
     unsigned long long nq = 0, npq = 0;
-    for (int j = 0; j < jend; ++j) {  // uniform over the work-group
-        const int sk = a.ket_start + j;
-        const int tk = a.pr.tonto[sk];
-        const double mIk = maxI[j];
-        const bool bhi = tb >= tk;
-        const double Iab = bhi ? mIb : mIk, Icd = bhi ? mIk : mIb;
-        if (Iab * IP_max < a.cutoff) continue;
-        const int kr = a.pr.sh_p[sk], ks = a.pr.sh_q[sk];
-        const double Pmax_k = a.Pmax[tk];
-        const double JP = bhi ? fmax(Pmax_b, Pmax_k) : fmax(Pmax_k, Pmax_b);
-        const double KP = fmax(fmax(a.Pmax[tonto_pair(bp, kr)], a.Pmax[tonto_pair(bp, ks)]),
-                               fmax(a.Pmax[tonto_pair(bq, kr)], a.Pmax[tonto_pair(bq, ks)]));
-        const bool skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
-        const bool skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
-        if (skipJ && skipK) continue;
-        double f = (bp == bq) ? 0.5 : 1.0;
-        if (kr == ks) f *= 0.5;
-        if (sb == sk) f *= 0.5;
-        const int akr = a.pr.atom_p[sk], aks = a.pr.atom_q[sk];
-        const bool any_same = atp == atq || akr == aks || atp == akr || atp == aks ||
-                              atq == akr || atq == aks;
-        const PairView bv = make_view(a.pr, a.pm, sb, !any_same);
-        const PairView kv = make_view(a.pr, a.pm, sk, !any_same);
-        const int fr = a.pr.bf_p[sk], fs = a.pr.bf_q[sk];
+    for (int j = 0; j < b.jend; ++j) {  // uniform over the work-group
+        KetTest t;
+        if (!test_ket(a, b, j, t)) continue;
+        const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
+        const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
+        const double f = t.f;
+        const int fr = a.pr.bf_p[t.sk], fs = a.pr.bf_q[t.sk];
         // stage the ket-dependent density blocks
-        for (int t = lane; t < NRS; t += 64) L.Prs[t] = P[(size_t)(fr + t / NS) * nbf + fs + t % NS];
-        for (int t = lane; t < NS * NQ; t += 64) L.Psq[t] = P[(size_t)(fs + t / NQ) * nbf + fq + t % NQ];
-        for (int t = lane; t < NR * NQ; t += 64) L.Prq[t] = P[(size_t)(fr + t / NQ) * nbf + fq + t % NQ];
-        for (int t = lane; t < NS * NP; t += 64) L.Psp[t] = P[(size_t)(fs + t / NP) * nbf + fp + t % NP];
-        for (int t = lane; t < NR * NP; t += 64) L.Prp[t] = P[(size_t)(fr + t / NP) * nbf + fp + t % NP];
+        for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
+        for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fs + u / NQ) * ld + fq + u % NQ];
+        for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fr + u / NQ) * ld + fq + u % NQ];
+        for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fs + u / NP) * ld + fp + u % NP];
+        for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fr + u / NP) * ld + fp + u % NP];
         const double *I = wave_eri<LP, LQ, LR, LS>(bv, kv, L);
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
-        if (!skipJ) {
-            for (int t = lane; t < NRS; t += 64) {
+        if (!t.skipJ) {
+            for (int u = lane; u < NRS; u += 64) {
                 double acc = 0.0;
-                for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + t] * L.Ppq[pq];
-                unsafeAtomicAdd(&a.X[(size_t)(fr + t / NS) * nbf + fs + t % NS], f * acc);
+                for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + u] * L.Ppq[pq];
+                unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
             }
 #pragma unroll
             for (int m = 0; m < MJ; ++m) {
@@ -551,33 +569,33 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
                 }
             }
         }
-        if (!skipK) {
+        if (!t.skipK) {
             constexpr int N1 = NP * NR, N2 = N1 + NP * NS, N3 = N2 + NQ * NR, N4 = N3 + NQ * NS;
-            for (int t = lane; t < N4; t += 64) {
+            for (int u = lane; u < N4; u += 64) {
                 double acc = 0.0;
                 size_t dst;
-                if (t < N1) {  // K(p,r) += I P(s,q)
-                    const int p = t / NR, r = t % NR;
+                if (u < N1) {  // K(p,r) += I P(s,q)
+                    const int p = u / NR, r = u % NR;
                     for (int q = 0; q < NQ; ++q)
                         for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psq[s * NQ + q];
-                    dst = (size_t)(fp + p) * nbf + fr + r;
-                } else if (t < N2) {  // K(p,s) += I P(r,q)
-                    const int p = (t - N1) / NS, s = (t - N1) % NS;
+                    dst = (size_t)(fp + p) * ld + fr + r;
+                } else if (u < N2) {  // K(p,s) += I P(r,q)
+                    const int p = (u - N1) / NS, s = (u - N1) % NS;
                     for (int q = 0; q < NQ; ++q)
                         for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prq[r * NQ + q];
-                    dst = (size_t)(fp + p) * nbf + fs + s;
-                } else if (t < N3) {  // K(q,r) += I P(s,p)
-                    const int q = (t - N2) / NR, r = (t - N2) % NR;
+                    dst = (size_t)(fp + p) * ld + fs + s;
+                } else if (u < N3) {  // K(q,r) += I P(s,p)
+                    const int q = (u - N2) / NR, r = (u - N2) % NR;
                     for (int p = 0; p < NP; ++p)
                         for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psp[s * NP + p];
-                    dst = (size_t)(fq + q) * nbf + fr + r;
+                    dst = (size_t)(fq + q) * ld + fr + r;
                 } else {  // K(q,s) += I P(r,p)
-                    const int q = (t - N3) / NS, s = (t - N3) % NS;
+                    const int q = (u - N3) / NS, s = (u - N3) % NS;
                     for (int p = 0; p < NP; ++p)
                         for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prp[r * NP + p];
-                    dst = (size_t)(fq + q) * nbf + fs + s;
+                    dst = (size_t)(fq + q) * ld + fs + s;
                 }
-                unsafeAtomicAdd(&a.Y[dst], f * acc);
+                unsafeAtomicAdd(&b.Y[dst], f * acc);
             }
         }
         __syncthreads();
@@ -586,7 +604,7 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
     for (int m = 0; m < MJ; ++m) {
         const int pq = lane + 64 * m;
         if (pq < NPQ && Jpq[m] != 0.0)
-            unsafeAtomicAdd(&a.X[(size_t)(fp + pq / NQ) * nbf + fq + pq % NQ], Jpq[m]);
+            unsafeAtomicAdd(&b.X[(size_t)(fp + pq / NQ) * ld + fq + pq % NQ], Jpq[m]);
     }
     if (lane == 0 && nq) {
         atomicAdd(&a.counters[3 * a.counter_slot + 0], nq);
@@ -614,8 +632,11 @@ __global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims 
 }
 
 // Launchers defined in the per-class translation units (jk_class.hip)
+
+// Launchers defined in the per-class translation units (jk_class.hip).  variant 0: one
+// lane per quartet (light classes only); variant 1: one wavefront per quartet.
 #define TGPU_DECLARE_JK(LP, LQ, LR, LS, H) \
-    void tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, bool mixed, hipStream_t s);
+    void tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, int variant, bool mixed, hipStream_t s);
 TGPU_FOR_EACH_CLASS(TGPU_DECLARE_JK)
 #undef TGPU_DECLARE_JK
 #define TGPU_DECLARE_SCHWARZ(LP, LQ, H) \

@@ -1,22 +1,21 @@
 // libtonto_gpu: MI355X (gfx950) direct-SCF J/K build behind the C ABI of
 // include/tonto_gpu.h.  Drop-in for Tonto's MOLECULE.FOCK:make_r_JK_engine
-// (foofiles/molecule.fock.foo:1365-1505).
+// (foofiles/molecule.fock.foo:1365-1505), for one molecule or a batch of independent
+// molecules built in the same launches.
 //
-// Data layout in HBM (built once per geometry by tgpu_create / tgpu_schwarz):
-//   * shell pairs in Tonto's order are binned by angular-momentum class (lp >= lq,
-//     class = lp(lp+1)/2 + lq) and, after the Schwarz table is known, sorted inside
-//     each bin by max_I descending; per-pair fields are structure-of-arrays;
+// Data layout in HBM (built once per geometry by tgpu_create[_batch] / tgpu_schwarz):
+//   * every molecule's shell pairs (Tonto's order, a >= b) are binned by
+//     angular-momentum class (lp >= lq, class = lp(lp+1)/2 + lq); inside a bin the pairs
+//     are grouped by molecule and, once the Schwarz table is known, sorted by max_I
+//     descending; per-pair fields are structure-of-arrays;
 //   * primitive pairs (zeta, prefactor, product centre) are SoA, each shell pair owning
 //     two contiguous runs: the full list and the list with Tonto's 1e-6 primitive-pair
-//     skip applied (shell1quartet.foo:229-273, 374-416).
-// One J/K build = P_max per shell pair -> global maxima -> one launch per class pair
-// (bra class >= ket class) -> post-processing J = 2(X + X^T), K = Y + Y^T.
-// In the class-pair launch one 64-lane wavefront owns one bra pair and its lanes walk
-// the Schwarz-sorted ket pairs of the other bin up to the exact Schwarz frontier;
-// every lane applies Tonto's screening tests bit for bit (molecule.fock.foo:1412-1472,
-// Schwarz_test 3497-3536), evaluates the contracted quartet (Rys + HRR) and scatters
-// into J and K.  J(bra) partial sums stay in registers and are wavefront-reduced once
-// per bra pair; the remaining J and K updates are FP64 hardware atomics.
+//     skip applied (shell1quartet.foo:229-273, 374-416);
+//   * matrices (P, J, K and the accumulators X, Y) are packed molecule after molecule,
+//     each n_bf x n_bf.
+// One J/K build = P_max per shell pair -> per-molecule maxima -> one launch per class
+// pair (bra class >= ket class) -> post-processing J = 2(X + X^T), K = Y + Y^T.  The
+// sequence can be captured once into a hipGraph and replayed (TGPU_JK_GRAPH).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,42 +28,81 @@
 
 #include "tgpu_device.h"
 
-// P_max(ab) = max |P(a-block, b-block)| per Tonto shell pair (molecule.fock.foo:3601-3633)
-__global__ void pmax_kernel(const double *__restrict__ P, int nbf, int nshell,
-                            const int *__restrict__ sh_l, const int *__restrict__ sh_bf,
-                            double *__restrict__ Pmax)
+static constexpr int NCLS = TGPU_NPAIRCLASS;
+static constexpr int NQCLS = NCLS * (NCLS + 1) / 2;
+
+// ----------------------------------------------------------------------------------
+// Small kernels
+// ----------------------------------------------------------------------------------
+struct BatchArgs {
+    int n_mol;
+    const long long *pair_off;   // [n_mol+1] Tonto pairs of molecule m start here
+    const int *shell_off;        // [n_mol+1]
+    const int *atom_off;         // [n_mol+1]
+    const long long *mat_off;    // [n_mol]
+    const int *nbf;              // [n_mol]
+    const int *sh_l, *sh_bf;     // global shell index -> l, first basis function (local)
+};
+
+__device__ __forceinline__ int mol_of(const long long *off, int n, long long g)
 {
-    const long long npair = (long long)nshell * (nshell + 1) / 2;
-    for (long long ab = blockIdx.x * (long long)blockDim.x + threadIdx.x; ab < npair;
-         ab += (long long)gridDim.x * blockDim.x) {
-        int x = (int)((sqrt(8.0 * (double)ab + 1.0) - 1.0) * 0.5);
-        while ((long long)x * (x + 1) / 2 > ab) --x;
-        while ((long long)(x + 1) * (x + 2) / 2 <= ab) ++x;
-        const int y = (int)(ab - (long long)x * (x + 1) / 2);
-        const int fa = sh_bf[x], na = ncart(sh_l[x]), fb = sh_bf[y], nb = ncart(sh_l[y]);
-        double m = 0.0;
+    int lo = 0, hi = n;  // last m with off[m] <= g
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void decode_pair(long long ab, int &x, int &y)
+{
+    x = (int)((sqrt(8.0 * (double)ab + 1.0) - 1.0) * 0.5);
+    while ((long long)x * (x + 1) / 2 > ab) --x;
+    while ((long long)(x + 1) * (x + 2) / 2 <= ab) ++x;
+    y = (int)(ab - (long long)x * (x + 1) / 2);
+}
+
+// P_max(ab) = max |P(a-block, b-block)| per Tonto shell pair (molecule.fock.foo:3601-3633)
+__global__ void pmax_kernel(const double *__restrict__ P, BatchArgs b, double *__restrict__ Pmax)
+{
+    const long long npair = b.pair_off[b.n_mol];
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < npair;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int m = mol_of(b.pair_off, b.n_mol, g);
+        int x, y;
+        decode_pair(g - b.pair_off[m], x, y);
+        const int s0 = b.shell_off[m], ld = b.nbf[m];
+        const double *Pm = P + b.mat_off[m];
+        const int fa = b.sh_bf[s0 + x], na = ncart(b.sh_l[s0 + x]);
+        const int fb = b.sh_bf[s0 + y], nb = ncart(b.sh_l[s0 + y]);
+        double v = 0.0;
         for (int i = 0; i < na; ++i)
-            for (int j = 0; j < nb; ++j) m = fmax(m, fabs(P[(size_t)(fa + i) * nbf + fb + j]));
-        Pmax[ab] = m;
+            for (int j = 0; j < nb; ++j) v = fmax(v, fabs(Pm[(size_t)(fa + i) * ld + fb + j]));
+        Pmax[g] = v;
     }
 }
 
-// scal[0] = P_max = max over pairs; scal[1] = IP_max = max(max_I) * P_max (molecule.fock.foo:1401-1402)
-__global__ __launch_bounds__(1024) void global_max_kernel(const double *__restrict__ v, long long n,
-                                                          double maxI_all, double *scal)
+// per molecule m: scal[2m] = P_max = max over its pairs, scal[2m+1] = IP_max =
+// max(max_I) * P_max (molecule.fock.foo:1401-1402).  One block per molecule.
+__global__ __launch_bounds__(256) void mol_max_kernel(const double *__restrict__ Pmax, BatchArgs b,
+                                                      const double *__restrict__ maxI_mol,
+                                                      double *__restrict__ scal)
 {
-    __shared__ double red[1024];
-    double m = 0.0;
-    for (long long i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, v[i]);
-    red[threadIdx.x] = m;
+    __shared__ double red[256];
+    const int m = blockIdx.x;
+    double v = 0.0;
+    for (long long g = b.pair_off[m] + threadIdx.x; g < b.pair_off[m + 1]; g += blockDim.x)
+        v = fmax(v, Pmax[g]);
+    red[threadIdx.x] = v;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        scal[0] = red[0];
-        scal[1] = maxI_all * red[0];
+        scal[2 * m] = red[0];
+        scal[2 * m + 1] = maxI_mol[m] * red[0];
     }
 }
 
@@ -82,18 +120,20 @@ __global__ void fill_zero_u64_kernel(unsigned long long *a, int n)
 }
 
 // Tonto's post-processing: weight_diagonal_blocks(J,2), J = 2J, blockwise_symmetric_fold(K),
-// symmetric_reflect (molecule.fock.foo:1485-1500).  With the contributions kept in X and Y
-// (each unique quartet scattered once, with its permutation factor) these are
-// J = 2 (X + X^T) and K = Y + Y^T.
-__global__ void post_kernel(const double *__restrict__ X, const double *__restrict__ Y, int n,
+// symmetric_reflect (molecule.fock.foo:1485-1500).  With every unique quartet scattered
+// once with its permutation factor into X and Y these are J = 2 (X + X^T), K = Y + Y^T.
+// blockIdx.y = molecule.
+__global__ void post_kernel(const double *__restrict__ X, const double *__restrict__ Y, BatchArgs b,
                             double *__restrict__ J, double *__restrict__ K)
 {
-    const size_t nn = (size_t)n * n;
+    const int m = blockIdx.y;
+    const int n = b.nbf[m];
+    const size_t nn = (size_t)n * n, mo = b.mat_off[m];
     for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < nn;
          idx += (size_t)gridDim.x * blockDim.x) {
         const size_t i = idx / n, j = idx % n, t = j * n + i;
-        J[idx] = 2.0 * (X[idx] + X[t]);
-        K[idx] = Y[idx] + Y[t];
+        J[mo + idx] = 2.0 * (X[mo + idx] + X[mo + t]);
+        K[mo + idx] = Y[mo + idx] + Y[mo + t];
     }
 }
 
@@ -114,22 +154,24 @@ __device__ void os1d(double PA, double PB, double hp, int la, int lb, double S[6
 }
 
 struct OneEArgs {
-    int nshell, natom, nbf;
-    const int *sh_l, *sh_bf, *sh_atom, *sh_off, *sh_np, *cx, *cy, *cz;  // cx..: component tables [l][i]
+    BatchArgs b;
+    const int *sh_atom, *sh_off, *sh_np, *cx, *cy, *cz;  // cx..: component tables [l][i]
     const double *exps, *coefs, *xyz, *Z;
     double *S, *T, *V;
 };
 
 __global__ void one_e_kernel(OneEArgs a)
 {
-    const long long npair = (long long)a.nshell * (a.nshell + 1) / 2;
-    const long long ab = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (ab >= npair) return;
-    int x = (int)((sqrt(8.0 * (double)ab + 1.0) - 1.0) * 0.5);
-    while ((long long)x * (x + 1) / 2 > ab) --x;
-    while ((long long)(x + 1) * (x + 2) / 2 <= ab) ++x;
-    const int y = (int)(ab - (long long)x * (x + 1) / 2);
-    const int la = a.sh_l[x], lb = a.sh_l[y], na = ncart(la), nb = ncart(lb);
+    const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (g >= a.b.pair_off[a.b.n_mol]) return;
+    const int m = mol_of(a.b.pair_off, a.b.n_mol, g);
+    int x, y;
+    decode_pair(g - a.b.pair_off[m], x, y);
+    x += a.b.shell_off[m];
+    y += a.b.shell_off[m];
+    const int nbf = a.b.nbf[m];
+    const size_t mo = a.b.mat_off[m];
+    const int la = a.b.sh_l[x], lb = a.b.sh_l[y], na = ncart(la), nb = ncart(lb);
     const double *A = a.xyz + 3 * a.sh_atom[x], *B = a.xyz + 3 * a.sh_atom[y];
     double sb[10][10], tb[10][10], vb[10][10];
     for (int i = 0; i < 10; ++i)
@@ -165,7 +207,7 @@ __global__ void one_e_kernel(OneEArgs a)
                     sb[i][j] += pre * s3[0] * s3[1] * s3[2];
                     tb[i][j] += pre * (t3[0] * s3[1] * s3[2] + s3[0] * t3[1] * s3[2] + s3[0] * s3[1] * t3[2]);
                 }
-            for (int C = 0; C < a.natom; ++C) {
+            for (int C = a.b.atom_off[m]; C < a.b.atom_off[m + 1]; ++C) {
                 const double *Cc = a.xyz + 3 * C;
                 double PC2 = 0.0;
                 for (int t = 0; t < 3; ++t) PC2 += (Pc[t] - Cc[t]) * (Pc[t] - Cc[t]);
@@ -197,10 +239,10 @@ __global__ void one_e_kernel(OneEArgs a)
                 }
             }
         }
-    const int fa = a.sh_bf[x], fb = a.sh_bf[y];
+    const int fa = a.b.sh_bf[x], fb = a.b.sh_bf[y];
     for (int i = 0; i < na; ++i)
         for (int j = 0; j < nb; ++j) {
-            const size_t ij = (size_t)(fa + i) * a.nbf + fb + j, ji = (size_t)(fb + j) * a.nbf + fa + i;
+            const size_t ij = mo + (size_t)(fa + i) * nbf + fb + j, ji = mo + (size_t)(fb + j) * nbf + fa + i;
             a.S[ij] = a.S[ji] = sb[i][j];
             a.T[ij] = a.T[ji] = tb[i][j];
             a.V[ij] = a.V[ji] = vb[i][j];
@@ -210,44 +252,59 @@ __global__ void one_e_kernel(OneEArgs a)
 // ----------------------------------------------------------------------------------
 // Host side
 // ----------------------------------------------------------------------------------
-static constexpr int NCLS = TGPU_NPAIRCLASS;
-static constexpr int NQCLS = NCLS * (NCLS + 1) / 2;
-
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
 };
 
+static const bool kHeavy[NQCLS] = {
+#define TGPU_HEAVY_FLAG(LP, LQ, LR, LS, H) H,
+    TGPU_FOR_EACH_CLASS(TGPU_HEAVY_FLAG)
+#undef TGPU_HEAVY_FLAG
+};
+
 struct tgpu_ctx {
     std::string err;
     int device = 0;
-    hipStream_t stream = nullptr;
-    int n_atom = 0, n_shell = 0, n_bf = 0;
-    long long n_pairs = 0;
+    hipStream_t stream = nullptr;     // caller's stream
+    hipStream_t gstream = nullptr;    // private stream for graph capture / replay
+    // molecules
+    int n_mol = 0;
+    std::vector<int> atom_off, shell_off, nbf;
+    std::vector<long long> pair_off, mat_off;
+    long long n_elem = 0, n_pairs = 0;
+    // global atoms and shells
     std::vector<double> xyz, Z, exps, coefs;
-    std::vector<int> sh_l, sh_atom, sh_np, sh_off, sh_bf;
+    std::vector<int> sh_l, sh_atom, sh_np, sh_off, sh_bf, sh_mol;
     double cut_schwarz = 1e-9, cut_jp = 1e-9, cut_kp = 1e-9, cut_pair = 1e-6, fp32_thr = 1e-6;
-    // host pair tables in device order (binned by class; sorted by max_I after Schwarz)
+    // pair tables in device order: class bin, molecule, (after Schwarz) max_I descending
     int bin_start[NCLS + 1] = {0};
-    std::vector<int> tonto, sp, sq, bfp, bfq, atp, atq, off_full, n_full, off_skip, n_skip;
+    std::vector<int> seg_start, seg_count;  // [n_mol * NCLS]
+    std::vector<int> tonto, sp, sq, bfp, bfq, atp, atq, pmol, off_full, n_full, off_skip, n_skip;
     std::vector<double> A, AB, maxI;
     std::vector<double> pz, pK, pPx, pPy, pPz;
-    bool have_schwarz = false, sorted = false;
-    double maxI_all = 0.0;
-    std::vector<double> maxI_tonto;
+    bool have_schwarz = false;
+    std::vector<double> maxI_mol, maxI_tonto;
+    int variant[NQCLS];
     // device
-    DBuf<int> d_tonto, d_sp, d_sq, d_bfp, d_bfq, d_atp, d_atq, d_off_full, d_n_full, d_off_skip,
-        d_n_skip, d_sh_l, d_sh_bf;
-    DBuf<double> d_A, d_AB, d_maxI, d_pz, d_pK, d_pPx, d_pPy, d_pPz;
+    DBuf<int> d_tonto, d_sp, d_sq, d_bfp, d_bfq, d_atp, d_atq, d_pmol, d_off_full, d_n_full,
+        d_off_skip, d_n_skip, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
+        d_atom_off;
+    DBuf<long long> d_pair_off, d_mat_off;
+    DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol, d_pz, d_pK, d_pPx, d_pPy, d_pPz;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
     DBuf<unsigned long long> d_counters;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
     bool timed = false;
-    // optional per-class-pair timing (tgpu_set_profiling)
     bool profiling = false;
     std::vector<hipEvent_t> cls_ev;  // 2 per class pair
     std::vector<char> cls_launched;
+    // captured build
+    hipGraphExec_t gexec = nullptr;
+    const void *g_key[3] = {nullptr, nullptr, nullptr};
+    int g_flags = -1;
+    std::vector<int> g_variant;
 };
 
 #define HIPCHK(ctx, call)                                                              \
@@ -288,6 +345,13 @@ static void dfree(DBuf<T> &b)
     b.n = 0;
 }
 
+static void drop_graph(tgpu_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_flags = -1;
+}
+
 // Primitive-pair data of one Tonto shell pair (a >= b): full list, and the list with
 // pairs exp(-a b R^2/(a+b)) < pair_cutoff dropped.  Same arithmetic order as
 // SHELL1PAIR:make_precalculated_data (shell1pair.foo:128-143) and SHELL1QUARTET:set_ab_new
@@ -299,11 +363,13 @@ static void build_prims(tgpu_ctx *c)
     c->off_full.assign(np, 0); c->n_full.assign(np, 0);
     c->off_skip.assign(np, 0); c->n_skip.assign(np, 0);
     for (size_t s = 0; s < np; ++s) {
+        const int m = c->pmol[s], s0 = c->shell_off[m];
         const int ab = c->tonto[s];
         int a = (int)((std::sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
         while (a * (a + 1) / 2 > ab) --a;
         while ((a + 1) * (a + 2) / 2 <= ab) ++a;
-        const int b = ab - a * (a + 1) / 2;
+        const int b = ab - a * (a + 1) / 2 + s0;
+        a += s0;
         const double *PA = &c->xyz[3 * c->sh_atom[a]], *PB = &c->xyz[3 * c->sh_atom[b]];
         const bool same = c->sh_atom[a] == c->sh_atom[b];
         const double ABv[3] = {PA[0] - PB[0], PA[1] - PB[1], PA[2] - PB[2]};
@@ -349,15 +415,18 @@ static int upload_pairs(tgpu_ctx *c)
     if ((rc = dupload(c, c->d_tonto, c->tonto)) || (rc = dupload(c, c->d_sp, c->sp)) ||
         (rc = dupload(c, c->d_sq, c->sq)) || (rc = dupload(c, c->d_bfp, c->bfp)) ||
         (rc = dupload(c, c->d_bfq, c->bfq)) || (rc = dupload(c, c->d_atp, c->atp)) ||
-        (rc = dupload(c, c->d_atq, c->atq)) || (rc = dupload(c, c->d_off_full, c->off_full)) ||
-        (rc = dupload(c, c->d_n_full, c->n_full)) || (rc = dupload(c, c->d_off_skip, c->off_skip)) ||
-        (rc = dupload(c, c->d_n_skip, c->n_skip)) || (rc = dupload(c, c->d_A, c->A)) ||
-        (rc = dupload(c, c->d_AB, c->AB)) || (rc = dupload(c, c->d_pz, c->pz)) ||
-        (rc = dupload(c, c->d_pK, c->pK)) || (rc = dupload(c, c->d_pPx, c->pPx)) ||
-        (rc = dupload(c, c->d_pPy, c->pPy)) || (rc = dupload(c, c->d_pPz, c->pPz)))
+        (rc = dupload(c, c->d_atq, c->atq)) || (rc = dupload(c, c->d_pmol, c->pmol)) ||
+        (rc = dupload(c, c->d_off_full, c->off_full)) || (rc = dupload(c, c->d_n_full, c->n_full)) ||
+        (rc = dupload(c, c->d_off_skip, c->off_skip)) || (rc = dupload(c, c->d_n_skip, c->n_skip)) ||
+        (rc = dupload(c, c->d_A, c->A)) || (rc = dupload(c, c->d_AB, c->AB)) ||
+        (rc = dupload(c, c->d_pz, c->pz)) || (rc = dupload(c, c->d_pK, c->pK)) ||
+        (rc = dupload(c, c->d_pPx, c->pPx)) || (rc = dupload(c, c->d_pPy, c->pPy)) ||
+        (rc = dupload(c, c->d_pPz, c->pPz)) || (rc = dupload(c, c->d_seg_start, c->seg_start)) ||
+        (rc = dupload(c, c->d_seg_count, c->seg_count)))
         return rc;
     if (c->maxI.size() == c->tonto.size()) rc = dupload(c, c->d_maxI, c->maxI);
     else rc = dalloc(c, c->d_maxI, c->tonto.size());
+    drop_graph(c);
     return rc;
 }
 
@@ -365,7 +434,7 @@ static DevPairs dev_pairs(const tgpu_ctx *c)
 {
     DevPairs d;
     d.tonto = c->d_tonto.p; d.sh_p = c->d_sp.p; d.sh_q = c->d_sq.p; d.bf_p = c->d_bfp.p;
-    d.bf_q = c->d_bfq.p; d.atom_p = c->d_atp.p; d.atom_q = c->d_atq.p;
+    d.bf_q = c->d_bfq.p; d.atom_p = c->d_atp.p; d.atom_q = c->d_atq.p; d.mol = c->d_pmol.p;
     d.off_full = c->d_off_full.p; d.n_full = c->d_n_full.p; d.off_skip = c->d_off_skip.p;
     d.n_skip = c->d_n_skip.p; d.A = c->d_A.p; d.AB = c->d_AB.p; d.maxI = c->d_maxI.p;
     return d;
@@ -378,6 +447,23 @@ static DevPrims dev_prims(const tgpu_ctx *c)
     return d;
 }
 
+static DevMols dev_mols(const tgpu_ctx *c)
+{
+    DevMols d;
+    d.seg_start = c->d_seg_start.p; d.seg_count = c->d_seg_count.p; d.nbf = c->d_nbf.p;
+    d.mat_off = c->d_mat_off.p; d.pair_off = c->d_pair_off.p;
+    return d;
+}
+
+static BatchArgs batch_args(const tgpu_ctx *c)
+{
+    BatchArgs b;
+    b.n_mol = c->n_mol; b.pair_off = c->d_pair_off.p; b.shell_off = c->d_shell_off.p;
+    b.atom_off = c->d_atom_off.p; b.mat_off = c->d_mat_off.p; b.nbf = c->d_nbf.p;
+    b.sh_l = c->d_sh_l.p; b.sh_bf = c->d_sh_bf.p;
+    return b;
+}
+
 // Permute all per-pair host arrays by `perm` (new position -> old position).
 static void permute_pairs(tgpu_ctx *c, const std::vector<int> &perm)
 {
@@ -387,107 +473,165 @@ static void permute_pairs(tgpu_ctx *c, const std::vector<int> &perm)
             for (int k = 0; k < stride; ++k) v[i * stride + k] = old[(size_t)perm[i] * stride + k];
     };
     P(c->tonto, 1); P(c->sp, 1); P(c->sq, 1); P(c->bfp, 1); P(c->bfq, 1); P(c->atp, 1);
-    P(c->atq, 1); P(c->off_full, 1); P(c->n_full, 1); P(c->off_skip, 1); P(c->n_skip, 1);
-    P(c->A, 3); P(c->AB, 3);
+    P(c->atq, 1); P(c->pmol, 1); P(c->off_full, 1); P(c->n_full, 1); P(c->off_skip, 1);
+    P(c->n_skip, 1); P(c->A, 3); P(c->AB, 3);
     if (c->maxI.size() == perm.size()) P(c->maxI, 1);
 }
 
-extern "C" int tgpu_create(const tgpu_system *sys, tgpu_ctx **out)
+extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **out)
 {
-    if (!sys || !out || sys->n_atom <= 0 || sys->n_shell <= 0) return TGPU_E_ARG;
+    if (!out) return TGPU_E_ARG;
+    *out = nullptr;
+    if (!sys || n_mol <= 0) return TGPU_E_ARG;
     tgpu_ctx *c = new tgpu_ctx();
     *out = c;
     if (hipGetDevice(&c->device) != hipSuccess) {
         c->err = "no HIP device";
         return TGPU_E_HIP;
     }
-    c->n_atom = sys->n_atom;
-    c->n_shell = sys->n_shell;
-    c->xyz.assign(sys->atom_xyz, sys->atom_xyz + 3 * sys->n_atom);
-    c->Z.assign(sys->atom_Z, sys->atom_Z + sys->n_atom);
-    c->sh_l.assign(sys->shell_l, sys->shell_l + sys->n_shell);
-    c->sh_atom.assign(sys->shell_atom, sys->shell_atom + sys->n_shell);
-    c->sh_np.assign(sys->shell_nprim, sys->shell_nprim + sys->n_shell);
-    c->sh_off.resize(c->n_shell);
-    c->sh_bf.resize(c->n_shell);
-    int off = 0, bf = 0;
-    for (int s = 0; s < c->n_shell; ++s) {
-        if (c->sh_l[s] < 0 || c->sh_l[s] > TGPU_LMAX) {
-            c->err = "shell angular momentum " + std::to_string(c->sh_l[s]) +
-                     " exceeds this build's maximum " + std::to_string(TGPU_LMAX);
-            return TGPU_E_UNSUPPORTED;
-        }
-        if (c->sh_np[s] <= 0 || c->sh_atom[s] < 0 || c->sh_atom[s] >= c->n_atom) {
-            c->err = "invalid shell " + std::to_string(s);
+    c->n_mol = n_mol;
+    c->atom_off.assign(n_mol + 1, 0);
+    c->shell_off.assign(n_mol + 1, 0);
+    c->pair_off.assign(n_mol + 1, 0);
+    c->mat_off.assign(n_mol, 0);
+    c->nbf.assign(n_mol, 0);
+    int prim = 0;
+    for (int m = 0; m < n_mol; ++m) {
+        const tgpu_system &S = sys[m];
+        if (S.n_atom <= 0 || S.n_shell <= 0 || !S.atom_xyz || !S.atom_Z || !S.shell_l ||
+            !S.shell_atom || !S.shell_nprim || !S.exps || !S.coefs) {
+            c->err = "molecule " + std::to_string(m) + ": empty or incomplete tgpu_system";
             return TGPU_E_ARG;
         }
-        c->sh_off[s] = off;
-        c->sh_bf[s] = bf;
-        off += c->sh_np[s];
-        bf += ncart(c->sh_l[s]);
-    }
-    c->exps.assign(sys->exps, sys->exps + off);
-    c->coefs.assign(sys->coefs, sys->coefs + off);
-    c->n_bf = bf;
-    c->n_pairs = (long long)c->n_shell * (c->n_shell + 1) / 2;
-    // bin Tonto's pairs by class, keeping Tonto order within a bin
-    std::vector<std::vector<int>> bins(NCLS);
-    for (int a = 0; a < c->n_shell; ++a)
-        for (int b = 0; b <= a; ++b) {
-            const int lp = std::max(c->sh_l[a], c->sh_l[b]), lq = std::min(c->sh_l[a], c->sh_l[b]);
-            bins[pair_class(lp, lq)].push_back(a * (a + 1) / 2 + b);
+        const int a0 = (int)c->Z.size();
+        c->atom_off[m] = a0;
+        c->xyz.insert(c->xyz.end(), S.atom_xyz, S.atom_xyz + 3 * S.n_atom);
+        c->Z.insert(c->Z.end(), S.atom_Z, S.atom_Z + S.n_atom);
+        c->shell_off[m] = (int)c->sh_l.size();
+        int bf = 0, np = 0;
+        for (int s = 0; s < S.n_shell; ++s) {
+            const int l = S.shell_l[s];
+            if (l < 0 || l > TGPU_LMAX) {
+                c->err = "shell angular momentum " + std::to_string(l) +
+                         " exceeds this build's maximum " + std::to_string(TGPU_LMAX);
+                return TGPU_E_UNSUPPORTED;
+            }
+            if (S.shell_nprim[s] <= 0 || S.shell_atom[s] < 0 || S.shell_atom[s] >= S.n_atom) {
+                c->err = "molecule " + std::to_string(m) + ": invalid shell " + std::to_string(s);
+                return TGPU_E_ARG;
+            }
+            c->sh_l.push_back(l);
+            c->sh_atom.push_back(a0 + S.shell_atom[s]);
+            c->sh_np.push_back(S.shell_nprim[s]);
+            c->sh_off.push_back(prim + np);
+            c->sh_bf.push_back(bf);
+            c->sh_mol.push_back(m);
+            np += S.shell_nprim[s];
+            bf += ncart(l);
         }
-    c->tonto.clear();
+        c->exps.insert(c->exps.end(), S.exps, S.exps + np);
+        c->coefs.insert(c->coefs.end(), S.coefs, S.coefs + np);
+        prim += np;
+        c->nbf[m] = bf;
+        c->mat_off[m] = c->n_elem;
+        c->n_elem += (long long)bf * bf;
+        c->pair_off[m] = c->n_pairs;
+        c->n_pairs += (long long)S.n_shell * (S.n_shell + 1) / 2;
+    }
+    c->atom_off[n_mol] = (int)c->Z.size();
+    c->shell_off[n_mol] = (int)c->sh_l.size();
+    c->pair_off[n_mol] = c->n_pairs;
+    if (c->n_pairs > 0x7fffffffLL) {
+        c->err = "too many shell pairs for 32-bit pair indexing";
+        return TGPU_E_UNSUPPORTED;
+    }
+    // pairs: class bin, then molecule, then Tonto order
+    std::vector<std::vector<std::pair<int, int>>> bins(NCLS);  // (molecule, local Tonto index)
+    for (int m = 0; m < n_mol; ++m) {
+        const int s0 = c->shell_off[m], ns = c->shell_off[m + 1] - s0;
+        for (int a = 0; a < ns; ++a)
+            for (int b = 0; b <= a; ++b) {
+                const int la = c->sh_l[s0 + a], lb = c->sh_l[s0 + b];
+                bins[pair_class(std::max(la, lb), std::min(la, lb))].push_back({m, a * (a + 1) / 2 + b});
+            }
+    }
+    c->seg_start.assign((size_t)n_mol * NCLS, 0);
+    c->seg_count.assign((size_t)n_mol * NCLS, 0);
     for (int k = 0; k < NCLS; ++k) {
         c->bin_start[k] = (int)c->tonto.size();
-        c->tonto.insert(c->tonto.end(), bins[k].begin(), bins[k].end());
+        for (auto &pr : bins[k]) {
+            if (c->seg_count[(size_t)pr.first * NCLS + k] == 0)
+                c->seg_start[(size_t)pr.first * NCLS + k] = (int)c->tonto.size();
+            c->seg_count[(size_t)pr.first * NCLS + k]++;
+            c->pmol.push_back(pr.first);
+            c->tonto.push_back(pr.second);
+        }
     }
     c->bin_start[NCLS] = (int)c->tonto.size();
     const size_t np = c->tonto.size();
     c->sp.resize(np); c->sq.resize(np); c->bfp.resize(np); c->bfq.resize(np);
     c->atp.resize(np); c->atq.resize(np); c->A.resize(3 * np); c->AB.resize(3 * np);
     for (size_t s = 0; s < np; ++s) {
+        const int m = c->pmol[s], s0 = c->shell_off[m];
         const int ab = c->tonto[s];
         int a = (int)((std::sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
         while (a * (a + 1) / 2 > ab) --a;
         while ((a + 1) * (a + 2) / 2 <= ab) ++a;
         const int b = ab - a * (a + 1) / 2;
-        const int p = c->sh_l[a] >= c->sh_l[b] ? a : b, q = p == a ? b : a;
+        const int p = c->sh_l[s0 + a] >= c->sh_l[s0 + b] ? a : b, q = p == a ? b : a;
         c->sp[s] = p; c->sq[s] = q;
-        c->bfp[s] = c->sh_bf[p]; c->bfq[s] = c->sh_bf[q];
-        c->atp[s] = c->sh_atom[p]; c->atq[s] = c->sh_atom[q];
+        c->bfp[s] = c->sh_bf[s0 + p]; c->bfq[s] = c->sh_bf[s0 + q];
+        c->atp[s] = c->sh_atom[s0 + p]; c->atq[s] = c->sh_atom[s0 + q];
         for (int t = 0; t < 3; ++t) {
-            c->A[3 * s + t] = c->xyz[3 * c->sh_atom[p] + t];
-            c->AB[3 * s + t] = c->xyz[3 * c->sh_atom[p] + t] - c->xyz[3 * c->sh_atom[q] + t];
+            c->A[3 * s + t] = c->xyz[3 * c->sh_atom[s0 + p] + t];
+            c->AB[3 * s + t] = c->xyz[3 * c->sh_atom[s0 + p] + t] - c->xyz[3 * c->sh_atom[s0 + q] + t];
         }
     }
     build_prims(c);
+    for (int k = 0; k < NQCLS; ++k) c->variant[k] = kHeavy[k] ? 1 : 0;
     int rc;
     if ((rc = upload_pairs(c))) return rc;
-    if ((rc = dupload(c, c->d_sh_l, c->sh_l)) || (rc = dupload(c, c->d_sh_bf, c->sh_bf))) return rc;
-    const size_t nn = (size_t)c->n_bf * c->n_bf;
-    if ((rc = dalloc(c, c->d_X, nn)) || (rc = dalloc(c, c->d_Y, nn)) ||
-        (rc = dalloc(c, c->d_Pmax, (size_t)c->n_pairs)) || (rc = dalloc(c, c->d_scal, 4)) ||
-        (rc = dalloc(c, c->d_counters, 3 * NQCLS)))
+    if ((rc = dupload(c, c->d_sh_l, c->sh_l)) || (rc = dupload(c, c->d_sh_bf, c->sh_bf)) ||
+        (rc = dupload(c, c->d_nbf, c->nbf)) || (rc = dupload(c, c->d_shell_off, c->shell_off)) ||
+        (rc = dupload(c, c->d_atom_off, c->atom_off)) || (rc = dupload(c, c->d_pair_off, c->pair_off)) ||
+        (rc = dupload(c, c->d_mat_off, c->mat_off)))
+        return rc;
+    if ((rc = dalloc(c, c->d_X, (size_t)c->n_elem)) || (rc = dalloc(c, c->d_Y, (size_t)c->n_elem)) ||
+        (rc = dalloc(c, c->d_Pmax, (size_t)c->n_pairs)) || (rc = dalloc(c, c->d_scal, 2 * (size_t)n_mol)) ||
+        (rc = dalloc(c, c->d_maxI_mol, (size_t)n_mol)) || (rc = dalloc(c, c->d_counters, 3 * NQCLS)))
         return rc;
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
     return TGPU_OK;
+}
+
+extern "C" int tgpu_create(const tgpu_system *sys, tgpu_ctx **out)
+{
+    return tgpu_create_batch(sys, 1, out);
 }
 
 extern "C" int tgpu_destroy(tgpu_ctx *c)
 {
     if (!c) return TGPU_E_ARG;
+    drop_graph(c);
     dfree(c->d_tonto); dfree(c->d_sp); dfree(c->d_sq); dfree(c->d_bfp); dfree(c->d_bfq);
-    dfree(c->d_atp); dfree(c->d_atq); dfree(c->d_off_full); dfree(c->d_n_full);
+    dfree(c->d_atp); dfree(c->d_atq); dfree(c->d_pmol); dfree(c->d_off_full); dfree(c->d_n_full);
     dfree(c->d_off_skip); dfree(c->d_n_skip); dfree(c->d_sh_l); dfree(c->d_sh_bf);
-    dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_pz); dfree(c->d_pK);
-    dfree(c->d_pPx); dfree(c->d_pPy); dfree(c->d_pPz); dfree(c->d_X); dfree(c->d_Y);
+    dfree(c->d_seg_start); dfree(c->d_seg_count); dfree(c->d_nbf); dfree(c->d_shell_off);
+    dfree(c->d_atom_off); dfree(c->d_pair_off); dfree(c->d_mat_off);
+    dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_pz);
+    dfree(c->d_pK); dfree(c->d_pPx); dfree(c->d_pPy); dfree(c->d_pPz); dfree(c->d_X); dfree(c->d_Y);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
     dfree(c->d_counters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+    if (c->ev_out) (void)hipEventDestroy(c->ev_out);
     for (auto &e : c->cls_ev) (void)hipEventDestroy(e);
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
     delete c;
     return TGPU_OK;
 }
@@ -500,8 +644,23 @@ extern "C" const char *tgpu_last_error(const tgpu_ctx *c)
 extern "C" int tgpu_dims(const tgpu_ctx *c, int *n_bf, long long *n_pairs)
 {
     if (!c) return TGPU_E_ARG;
-    if (n_bf) *n_bf = c->n_bf;
-    if (n_pairs) *n_pairs = c->n_pairs;
+    if (n_bf) *n_bf = c->nbf[0];
+    if (n_pairs) *n_pairs = c->pair_off[1];
+    return TGPU_OK;
+}
+
+extern "C" int tgpu_batch_layout(const tgpu_ctx *c, int *n_mol, long long *n_elements,
+                                 long long *mat_off, int *n_bf, long long *pair_off)
+{
+    if (!c) return TGPU_E_ARG;
+    if (n_mol) *n_mol = c->n_mol;
+    if (n_elements) *n_elements = c->n_elem;
+    for (int m = 0; m < c->n_mol; ++m) {
+        if (mat_off) mat_off[m] = c->mat_off[m];
+        if (n_bf) n_bf[m] = c->nbf[m];
+    }
+    if (pair_off)
+        for (int m = 0; m <= c->n_mol; ++m) pair_off[m] = c->pair_off[m];
     return TGPU_OK;
 }
 
@@ -514,13 +673,15 @@ extern "C" int tgpu_set_stream(tgpu_ctx *c, void *s)
 
 extern "C" int tgpu_set_cutoffs(tgpu_ctx *c, double schwarz, double jp, double kp, double pair)
 {
-    if (!c || !(schwarz >= 0) || !(jp >= 0) || !(kp >= 0) || !(pair > 0)) {
-        if (c) c->err = "cutoffs must be non-negative (pair cutoff positive)";
+    if (!c) return TGPU_E_ARG;
+    if (!(schwarz >= 0) || !(jp >= 0) || !(kp >= 0) || !(pair > 0)) {
+        c->err = "cutoffs must be non-negative (pair cutoff positive)";
         return TGPU_E_ARG;
     }
     c->cut_schwarz = schwarz;
     c->cut_jp = jp;
     c->cut_kp = kp;
+    drop_graph(c);
     if (pair != c->cut_pair) {
         c->cut_pair = pair;
         build_prims(c);
@@ -534,9 +695,9 @@ extern "C" int tgpu_set_fp32_threshold(tgpu_ctx *c, double thr)
 {
     if (!c || !(thr >= 0)) return TGPU_E_ARG;
     c->fp32_thr = thr;
+    drop_graph(c);
     return TGPU_OK;
 }
-
 
 extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
 {
@@ -556,45 +717,63 @@ extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
         HIPCHK(c, hipMemcpyAsync(c->maxI.data(), c->d_maxI.p, c->maxI.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        // sort every bin by max_I descending (ties: Tonto order) and re-upload
+        // sort every (class, molecule) segment by max_I descending (ties: Tonto order)
         std::vector<int> perm(c->tonto.size());
         std::iota(perm.begin(), perm.end(), 0);
-        for (int k = 0; k < NCLS; ++k)
-            std::sort(perm.begin() + c->bin_start[k], perm.begin() + c->bin_start[k + 1],
-                      [&](int x, int y) {
-                          if (c->maxI[x] != c->maxI[y]) return c->maxI[x] > c->maxI[y];
-                          return c->tonto[x] < c->tonto[y];
-                      });
+        for (size_t k = 0; k < c->seg_start.size(); ++k) {
+            if (!c->seg_count[k]) continue;
+            auto b = perm.begin() + c->seg_start[k];
+            std::sort(b, b + c->seg_count[k], [&](int x, int y) {
+                if (c->maxI[x] != c->maxI[y]) return c->maxI[x] > c->maxI[y];
+                return c->tonto[x] < c->tonto[y];
+            });
+        }
         permute_pairs(c, perm);
         int rc = upload_pairs(c);
         if (rc) return rc;
         c->maxI_tonto.assign(c->n_pairs, 0.0);
-        c->maxI_all = 0.0;
+        c->maxI_mol.assign(c->n_mol, 0.0);
         for (size_t s = 0; s < c->tonto.size(); ++s) {
-            c->maxI_tonto[c->tonto[s]] = c->maxI[s];
-            c->maxI_all = std::max(c->maxI_all, c->maxI[s]);
+            const int m = c->pmol[s];
+            c->maxI_tonto[c->pair_off[m] + c->tonto[s]] = c->maxI[s];
+            c->maxI_mol[m] = std::max(c->maxI_mol[m], c->maxI[s]);
         }
+        if ((rc = dupload(c, c->d_maxI_mol, c->maxI_mol))) return rc;
         c->have_schwarz = true;
     }
     if (max_I) std::memcpy(max_I, c->maxI_tonto.data(), sizeof(double) * c->n_pairs);
     return TGPU_OK;
 }
 
-static void launch_jk(tgpu_ctx *c, const double *P, bool mixed, int slot, int X, int Y,
-                      void (*launcher)(const JKArgs &, bool, hipStream_t))
+typedef void (*jk_launcher)(const JKArgs &, int, bool, hipStream_t);
+static const jk_launcher kLaunch[NQCLS] = {
+#define TGPU_LAUNCHER(LP, LQ, LR, LS, H) tgpu_launch_jk_##LP##LQ##LR##LS,
+    TGPU_FOR_EACH_CLASS(TGPU_LAUNCHER)
+#undef TGPU_LAUNCHER
+};
+static const int kClassPair[NQCLS][2] = {
+#define TGPU_CLASSPAIR(LP, LQ, LR, LS, H) {pair_class(LP, LQ), pair_class(LR, LS)},
+    TGPU_FOR_EACH_CLASS(TGPU_CLASSPAIR)
+#undef TGPU_CLASSPAIR
+};
+
+static int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
+
+// Enqueue one complete build on stream s (no host synchronisation).
+static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, bool mixed,
+                          hipStream_t s, bool timing)
 {
-    if (c->profiling) c->cls_launched[slot] = 0;
+    const BatchArgs ba = batch_args(c);
+    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_X.p, c->n_elem);
+    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_Y.p, c->n_elem);
+    fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(c->d_counters.p, 3 * NQCLS);
+    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(P, ba, c->d_Pmax.p);
+    mol_max_kernel<<<c->n_mol, 256, 0, s>>>(c->d_Pmax.p, ba, c->d_maxI_mol.p, c->d_scal.p);
+    if (timing) (void)hipEventRecord(c->ev0, s);
     JKArgs a;
     a.pr = dev_pairs(c);
     a.pm = dev_prims(c);
-    a.bra_start = c->bin_start[X];
-    a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
-    a.ket_start = c->bin_start[Y];
-    a.n_ket = c->bin_start[Y + 1] - c->bin_start[Y];
-    if (a.n_bra == 0 || a.n_ket == 0) return;
-    a.same_bin = X == Y;
-    a.nbf = c->n_bf;
-    a.counter_slot = slot;
+    a.ml = dev_mols(c);
     a.P = P;
     a.Pmax = c->d_Pmax.p;
     a.scal = c->d_scal.p;
@@ -605,17 +784,29 @@ static void launch_jk(tgpu_ctx *c, const double *P, bool mixed, int slot, int X,
     a.X = c->d_X.p;
     a.Y = c->d_Y.p;
     a.counters = c->d_counters.p;
-    if (c->profiling) {
-        (void)hipEventRecord(c->cls_ev[2 * slot], c->stream);
-        launcher(a, mixed, c->stream);
-        (void)hipEventRecord(c->cls_ev[2 * slot + 1], c->stream);
-        c->cls_launched[slot] = 1;
-    } else {
-        launcher(a, mixed, c->stream);
+    for (int k = 0; k < NQCLS; ++k) {
+        const int X = kClassPair[k][0], Y = kClassPair[k][1];
+        a.bra_cls = X;
+        a.ket_cls = Y;
+        a.bra_start = c->bin_start[X];
+        a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
+        a.same_bin = X == Y;
+        a.counter_slot = k;
+        const int n_ket = c->bin_start[Y + 1] - c->bin_start[Y];
+        const bool prof = timing && c->profiling;
+        if (prof) c->cls_launched[k] = 0;
+        if (a.n_bra == 0 || n_ket == 0) continue;
+        if (prof) (void)hipEventRecord(c->cls_ev[2 * k], s);
+        kLaunch[k](a, c->variant[k], mixed, s);
+        if (prof) {
+            (void)hipEventRecord(c->cls_ev[2 * k + 1], s);
+            c->cls_launched[k] = 1;
+        }
     }
+    if (timing) (void)hipEventRecord(c->ev1, s);
+    post_kernel<<<dim3(grid_for(c->n_elem / c->n_mol + 1), c->n_mol), 256, 0, s>>>(c->d_X.p, c->d_Y.p,
+                                                                                   ba, J, K);
 }
-
-static int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
 
 extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, double *K, int flags)
 {
@@ -624,31 +815,46 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
         int rc = tgpu_schwarz(c, nullptr);
         if (rc) return rc;
     }
-    const size_t nn = (size_t)c->n_bf * c->n_bf;
-    fill_zero_kernel<<<grid_for(nn), 256, 0, c->stream>>>(c->d_X.p, nn);
-    fill_zero_kernel<<<grid_for(nn), 256, 0, c->stream>>>(c->d_Y.p, nn);
-    fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, c->stream>>>(c->d_counters.p, 3 * NQCLS);
-    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, c->stream>>>(
-        P, c->n_bf, c->n_shell, c->d_sh_l.p, c->d_sh_bf.p, c->d_Pmax.p);
-    global_max_kernel<<<1, 1024, 0, c->stream>>>(c->d_Pmax.p, c->n_pairs, c->maxI_all, c->d_scal.p);
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     const bool mixed = flags & TGPU_JK_MIXED;
-    int slot = 0;
-#define TGPU_JK_CASE(LP, LQ, LR, LS, H) \
-    launch_jk(c, P, mixed, slot++, pair_class(LP, LQ), pair_class(LR, LS), tgpu_launch_jk_##LP##LQ##LR##LS);
-    TGPU_FOR_EACH_CLASS(TGPU_JK_CASE)
-#undef TGPU_JK_CASE
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    c->timed = true;
-    post_kernel<<<grid_for(nn), 256, 0, c->stream>>>(c->d_X.p, c->d_Y.p, c->n_bf, J, K);
-    HIPCHK(c, hipGetLastError());
+    if (!(flags & TGPU_JK_GRAPH)) {
+        enqueue_build(c, P, J, K, mixed, c->stream, true);
+        c->timed = true;
+        HIPCHK(c, hipGetLastError());
+        return TGPU_OK;
+    }
+    // captured build: (re)capture when the pointers, flags or class variants changed
+    const std::vector<int> var(c->variant, c->variant + NQCLS);
+    if (!c->gexec || c->g_key[0] != P || c->g_key[1] != J || c->g_key[2] != K ||
+        c->g_flags != (flags & TGPU_JK_MIXED) || c->g_variant != var) {
+        drop_graph(c);
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
+        enqueue_build(c, P, J, K, mixed, c->gstream, false);
+        HIPCHK(c, hipStreamEndCapture(c->gstream, &g));
+        hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->gexec = nullptr;
+            c->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            return TGPU_E_HIP;
+        }
+        c->g_key[0] = P; c->g_key[1] = J; c->g_key[2] = K;
+        c->g_flags = flags & TGPU_JK_MIXED;
+        c->g_variant = var;
+    }
+    HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->gstream, c->ev_in, 0));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->gstream));
+    HIPCHK(c, hipEventRecord(c->ev_out, c->gstream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
+    c->timed = false;
     return TGPU_OK;
 }
 
 extern "C" int tgpu_make_r_JK(tgpu_ctx *c, const double *P, double *J, double *K, int flags)
 {
     if (!c || !P || !J || !K) return TGPU_E_ARG;
-    const size_t nn = (size_t)c->n_bf * c->n_bf;
+    const size_t nn = (size_t)c->n_elem;
     int rc;
     if ((rc = dalloc(c, c->d_hP, nn)) || (rc = dalloc(c, c->d_hJ, nn)) || (rc = dalloc(c, c->d_hK, nn)))
         return rc;
@@ -681,8 +887,6 @@ extern "C" int tgpu_get_stats(tgpu_ctx *c, tgpu_stats *st)
     return TGPU_OK;
 }
 
-// Per class-pair counters of the last build: out[3*k + {0,1,2}] = quartets, primitive
-// quartets, FP32 quartets for the k-th class pair in TGPU_FOR_EACH_CLASS order.
 extern "C" int tgpu_get_class_counters(tgpu_ctx *c, unsigned long long *out, int n)
 {
     if (!c || !out || n < 3 * NQCLS) return TGPU_E_ARG;
@@ -694,7 +898,6 @@ extern "C" int tgpu_get_class_counters(tgpu_ctx *c, unsigned long long *out, int
 
 extern "C" int tgpu_n_class_pairs(void) { return NQCLS; }
 
-// Per class-pair kernel timing with HIP events on the context's stream (off by default).
 extern "C" int tgpu_set_profiling(tgpu_ctx *c, int on)
 {
     if (!c) return TGPU_E_ARG;
@@ -707,14 +910,13 @@ extern "C" int tgpu_set_profiling(tgpu_ctx *c, int on)
     return TGPU_OK;
 }
 
-// ms[k] = device time of class pair k in the last build (0 if not launched).
 extern "C" int tgpu_get_class_times(tgpu_ctx *c, double *ms, int n)
 {
     if (!c || !ms || n < NQCLS) return TGPU_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < NQCLS; ++k) {
         ms[k] = 0.0;
-        if (c->profiling && c->cls_launched[k]) {
+        if (c->profiling && c->timed && c->cls_launched[k]) {
             float t = 0.f;
             HIPCHK(c, hipEventElapsedTime(&t, c->cls_ev[2 * k], c->cls_ev[2 * k + 1]));
             ms[k] = t;
@@ -723,10 +925,61 @@ extern "C" int tgpu_get_class_times(tgpu_ctx *c, double *ms, int n)
     return TGPU_OK;
 }
 
+extern "C" int tgpu_set_variant(tgpu_ctx *c, int k, int v)
+{
+    if (!c || k < 0 || k >= NQCLS || v < 0 || v > 1) return TGPU_E_ARG;
+    if (kHeavy[k] && v == 0) {
+        c->err = "class pair " + std::to_string(k) + " has only the wavefront form";
+        return TGPU_E_UNSUPPORTED;
+    }
+    c->variant[k] = v;
+    return TGPU_OK;
+}
+
+extern "C" int tgpu_get_variant(const tgpu_ctx *c, int k)
+{
+    if (!c || k < 0 || k >= NQCLS) return -1;
+    return c->variant[k];
+}
+
+// Autotuner: for every class pair that has both forms, time one build with each form on
+// the given density (device pointer) and keep the faster one.  ms_out (optional,
+// 2 * n_class_pairs doubles) receives the measured times of variant 0 and 1.
+extern "C" int tgpu_autotune(tgpu_ctx *c, const double *P, int flags, double *ms_out)
+{
+    if (!c || !P) return TGPU_E_ARG;
+    int rc;
+    if (!c->have_schwarz && (rc = tgpu_schwarz(c, nullptr))) return rc;
+    if ((rc = dalloc(c, c->d_hJ, (size_t)c->n_elem)) || (rc = dalloc(c, c->d_hK, (size_t)c->n_elem)))
+        return rc;
+    const bool was = c->profiling;
+    if ((rc = tgpu_set_profiling(c, 1))) return rc;
+    double t[2][NQCLS];
+    for (int v = 0; v < 2; ++v) {
+        for (int k = 0; k < NQCLS; ++k) c->variant[k] = kHeavy[k] ? 1 : v;
+        // one untimed pass so both forms start from warm caches
+        enqueue_build(c, P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream, false);
+        enqueue_build(c, P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream, true);
+        c->timed = true;
+        HIPCHK(c, hipGetLastError());
+        if ((rc = tgpu_get_class_times(c, t[v], NQCLS))) return rc;
+    }
+    for (int k = 0; k < NQCLS; ++k) {
+        c->variant[k] = kHeavy[k] ? 1 : (t[1][k] < t[0][k] ? 1 : 0);
+        if (ms_out) {
+            ms_out[2 * k] = kHeavy[k] ? -1.0 : t[0][k];
+            ms_out[2 * k + 1] = t[1][k];
+        }
+    }
+    c->profiling = was;
+    drop_graph(c);
+    return TGPU_OK;
+}
+
 extern "C" int tgpu_one_electron(tgpu_ctx *c, double *S, double *T, double *V)
 {
     if (!c || !S || !T || !V) return TGPU_E_ARG;
-    const size_t nn = (size_t)c->n_bf * c->n_bf;
+    const size_t nn = (size_t)c->n_elem;
     std::vector<int> cx(4 * 10, 0), cy(4 * 10, 0), cz(4 * 10, 0);
     // Tonto's Cartesian order (gaussian_data.foo:295-479) for l <= 3
     static const int tab[4][10][3] = {
@@ -756,9 +1009,9 @@ extern "C" int tgpu_one_electron(tgpu_ctx *c, double *S, double *T, double *V)
         return rc;
     }
     OneEArgs a;
-    a.nshell = c->n_shell; a.natom = c->n_atom; a.nbf = c->n_bf;
-    a.sh_l = c->d_sh_l.p; a.sh_bf = c->d_sh_bf.p; a.sh_atom = d_atom.p; a.sh_off = d_off.p;
-    a.sh_np = d_np.p; a.cx = d_cx.p; a.cy = d_cy.p; a.cz = d_cz.p; a.exps = d_e.p; a.coefs = d_c.p;
+    a.b = batch_args(c);
+    a.sh_atom = d_atom.p; a.sh_off = d_off.p; a.sh_np = d_np.p;
+    a.cx = d_cx.p; a.cy = d_cy.p; a.cz = d_cz.p; a.exps = d_e.p; a.coefs = d_c.p;
     a.xyz = d_xyz.p; a.Z = d_Z.p; a.S = d_S.p; a.T = d_T.p; a.V = d_V.p;
     one_e_kernel<<<(int)((c->n_pairs + 63) / 64), 64, 0, c->stream>>>(a);
     hipError_t e = hipGetLastError();

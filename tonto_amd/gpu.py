@@ -12,13 +12,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtonto_gpu.so")
 
 TGPU_JK_MIXED = 1
+TGPU_JK_GRAPH = 2
 
 # exported symbols, in the order of include/tonto_gpu.h (plus the two extensions)
 EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_set_stream",
            "tgpu_set_cutoffs", "tgpu_set_fp32_threshold", "tgpu_schwarz", "tgpu_make_r_JK",
            "tgpu_make_r_JK_device", "tgpu_get_stats", "tgpu_one_electron",
            "tgpu_get_class_counters", "tgpu_n_class_pairs", "tgpu_set_profiling",
-           "tgpu_get_class_times"]
+           "tgpu_get_class_times", "tgpu_create_batch", "tgpu_batch_layout", "tgpu_set_variant",
+           "tgpu_get_variant", "tgpu_autotune"]
 
 _lib = None
 
@@ -68,6 +70,12 @@ def lib():
         L.tgpu_n_class_pairs.restype = ctypes.c_int
         L.tgpu_set_profiling.argtypes = [vp, ctypes.c_int]
         L.tgpu_get_class_times.argtypes = [vp, dp, ctypes.c_int]
+        L.tgpu_create_batch.argtypes = [ctypes.POINTER(_System), ctypes.c_int, ctypes.POINTER(vp)]
+        L.tgpu_batch_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_longlong), vp, vp, vp]
+        L.tgpu_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.tgpu_get_variant.argtypes = [vp, ctypes.c_int]
+        L.tgpu_autotune.argtypes = [vp, vp, ctypes.c_int, vp]
         _lib = L
     return _lib
 
@@ -81,22 +89,36 @@ def _ip(a):
 
 
 class JKEngine:
-    """One molecule's context on the current HIP device (mirrors Tonto's
-    MOLECULE state consumed by make_r_JK_engine)."""
+    """A context on the current HIP device for one molecule, or for a list of
+    independent molecules built together (mirrors Tonto's MOLECULE state consumed by
+    make_r_JK_engine).  For a batch, matrices are packed molecule after molecule."""
 
     def __init__(self, mol, schwarz=1e-9, j_density=1e-9, k_density=1e-9, pair=1e-6,
                  fp32_threshold=1e-6):
+        mols = list(mol) if isinstance(mol, (list, tuple)) else [mol]
         self.mol = mol
-        a = mol.shell_arrays()
-        self._keep = a
-        self._sys = _System(mol.n_atom, mol.n_shell, _dp(a["atom_xyz"]), _dp(a["atom_Z"]),
-                            _ip(a["l"]), _ip(a["atom"]), _ip(a["nprim"]), _dp(a["exps"]),
-                            _dp(a["coefs"]))
+        self.mols = mols
+        self._keep = [m.shell_arrays() for m in mols]
+        arr = (_System * len(mols))()
+        for i, (m, a) in enumerate(zip(mols, self._keep)):
+            arr[i] = _System(m.n_atom, m.n_shell, _dp(a["atom_xyz"]), _dp(a["atom_Z"]),
+                             _ip(a["l"]), _ip(a["atom"]), _ip(a["nprim"]), _dp(a["exps"]),
+                             _dp(a["coefs"]))
+        self._sys = arr
         self._ctx = ctypes.c_void_p()
-        rc = lib().tgpu_create(ctypes.byref(self._sys), ctypes.byref(self._ctx))
+        rc = lib().tgpu_create_batch(arr, len(mols), ctypes.byref(self._ctx))
         self._check(rc)
-        self.n_bf = mol.n_bf
-        self.n_pairs = mol.n_shell * (mol.n_shell + 1) // 2
+        self.n_mol = len(mols)
+        self.n_bf = mols[0].n_bf
+        self.mat_off = np.zeros(self.n_mol, dtype=np.int64)
+        self.nbf = np.zeros(self.n_mol, dtype=np.int32)
+        self.pair_off = np.zeros(self.n_mol + 1, dtype=np.int64)
+        nm, ne = ctypes.c_int(), ctypes.c_longlong()
+        self._check(lib().tgpu_batch_layout(self._ctx, ctypes.byref(nm), ctypes.byref(ne),
+                                            self.mat_off.ctypes.data, self.nbf.ctypes.data,
+                                            self.pair_off.ctypes.data))
+        self.n_elements = ne.value
+        self.n_pairs = int(self.pair_off[-1])
         self._check(lib().tgpu_set_cutoffs(self._ctx, schwarz, j_density, k_density, pair))
         self._check(lib().tgpu_set_fp32_threshold(self._ctx, fp32_threshold))
 
@@ -124,20 +146,46 @@ class JKEngine:
         self._check(lib().tgpu_schwarz(self._ctx, out.ctypes.data))
         return out
 
-    def make_r_JK(self, P, mixed=False):
-        """Host arrays in, host arrays out (synchronous)."""
-        P = np.ascontiguousarray(P, dtype=np.float64)
-        n = self.n_bf
-        J, K = np.zeros((n, n)), np.zeros((n, n))
-        self._check(lib().tgpu_make_r_JK(self._ctx, _dp(P), _dp(J), _dp(K),
-                                         TGPU_JK_MIXED if mixed else 0))
-        return J, K
+    def pack(self, mats):
+        """Pack a list of per-molecule square matrices into the batch layout."""
+        return np.concatenate([np.ascontiguousarray(m, dtype=np.float64).ravel() for m in mats])
 
-    def make_r_JK_device(self, P_ptr, J_ptr, K_ptr, mixed=False):
+    def unpack(self, flat):
+        return [flat[o:o + n * n].reshape(n, n) for o, n in zip(self.mat_off, self.nbf)]
+
+    def make_r_JK(self, P, mixed=False):
+        """Host arrays in, host arrays out (synchronous).  For a batch, P is a list of
+        matrices (or the packed vector) and lists are returned."""
+        batch = self.n_mol > 1 or isinstance(P, (list, tuple))
+        Pf = self.pack(P) if isinstance(P, (list, tuple)) else np.ascontiguousarray(P, dtype=np.float64).ravel()
+        assert Pf.size == self.n_elements
+        J, K = np.zeros(self.n_elements), np.zeros(self.n_elements)
+        self._check(lib().tgpu_make_r_JK(self._ctx, _dp(Pf), _dp(J), _dp(K),
+                                         TGPU_JK_MIXED if mixed else 0))
+        if batch:
+            return self.unpack(J), self.unpack(K)
+        n = self.n_bf
+        return J.reshape(n, n), K.reshape(n, n)
+
+    def make_r_JK_device(self, P_ptr, J_ptr, K_ptr, mixed=False, graph=False):
         """Device pointers (e.g. torch tensor .data_ptr()), asynchronous on the stream."""
+        flags = (TGPU_JK_MIXED if mixed else 0) | (TGPU_JK_GRAPH if graph else 0)
         self._check(lib().tgpu_make_r_JK_device(self._ctx, ctypes.c_void_p(P_ptr),
                                                 ctypes.c_void_p(J_ptr), ctypes.c_void_p(K_ptr),
-                                                TGPU_JK_MIXED if mixed else 0))
+                                                flags))
+
+    def autotune(self, P_ptr, mixed=False):
+        n = lib().tgpu_n_class_pairs()
+        ms = np.zeros(2 * n)
+        self._check(lib().tgpu_autotune(self._ctx, ctypes.c_void_p(P_ptr),
+                                        TGPU_JK_MIXED if mixed else 0, ms.ctypes.data))
+        return ms.reshape(n, 2), self.variants()
+
+    def variants(self):
+        return [lib().tgpu_get_variant(self._ctx, k) for k in range(lib().tgpu_n_class_pairs())]
+
+    def set_variant(self, k, v):
+        self._check(lib().tgpu_set_variant(self._ctx, k, v))
 
     def stats(self):
         s = Stats()
@@ -161,7 +209,9 @@ class JKEngine:
         return out
 
     def one_electron(self):
-        n = self.n_bf
-        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        S, T, V = (np.zeros(self.n_elements) for _ in range(3))
         self._check(lib().tgpu_one_electron(self._ctx, _dp(S), _dp(T), _dp(V)))
-        return S, T, V
+        if self.n_mol > 1:
+            return self.unpack(S), self.unpack(T), self.unpack(V)
+        n = self.n_bf
+        return S.reshape(n, n), T.reshape(n, n), V.reshape(n, n)
