@@ -7,8 +7,9 @@ import time
 
 import torch
 
+from .basis import Molecule
 from .gpu import JKEngine
-from .scf import run_rhf
+from .scf import atomic_density, promolecule_guess, run_rhf
 
 
 class GPUSCF:
@@ -48,10 +49,49 @@ class GPUSCF:
                                      mixed=mode.endswith("_mixed"), graph=self.graph)
         return self._J.clone(), self._K.clone()
 
-    def run(self, precision="fp64", mixed_until=1e-4, incremental=False, log=None, **kw):
+    def atom_scf(self, symbol):
+        """Spherically averaged density of one atom of the molecule's basis, on the GPU."""
+        from .basis import ELEMENT_Z
+        atom = Molecule.from_angstrom([symbol], [[0.0, 0.0, 0.0]], self.mol.basis_name)
+        eng = JKEngine(atom)
+        S, T, V = eng.one_electron()
+        eng.schwarz()
+        n = atom.n_bf
+        Pd = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        Jd, Kd = torch.empty_like(Pd), torch.empty_like(Pd)
+
+        def jk(P, mode):
+            Pd.copy_(P)
+            eng.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+            eng.make_r_JK_device(Pd.data_ptr(), Jd.data_ptr(), Kd.data_ptr())
+            return Jd.clone(), Kd.clone()
+        return atomic_density(torch.tensor(T + V, device=self.device), torch.tensor(S, device=self.device),
+                              ELEMENT_Z[symbol], jk)
+
+    def guess(self, kind="promolecule"):
+        if kind == "core":
+            return None
+        return promolecule_guess(self.mol, self.atom_scf)
+
+    def jk_incremental(self, P, mode):
+        """Delta builds are screened with a 1000x tighter Schwarz cutoff than full builds:
+        the contributions a full build drops are bounded by the cutoff times |P|, a delta
+        build's by the cutoff times |dP|, and those errors would otherwise accumulate."""
+        if mode.startswith("delta"):
+            self.engine.set_cutoffs_scaled(1e-3)
+            try:
+                return self.jk(P, mode)
+            finally:
+                self.engine.set_cutoffs_scaled(1.0)
+        return self.jk(P, mode)
+
+    def run(self, precision="fp64", mixed_until=1e-4, incremental=False, guess="promolecule",
+            log=None, **kw):
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
-        r = run_rhf(self.h, self.S, self.mol.n_electrons, self.mol.nuclear_repulsion(), self.jk,
+        P0 = self.guess(guess)
+        r = run_rhf(self.h, self.S, self.mol.n_electrons, self.mol.nuclear_repulsion(),
+                    self.jk_incremental if incremental else self.jk, P0=P0,
                     mixed_until=mixed_until if precision == "mixed" else None,
                     incremental=incremental, log=log, **kw)
         torch.cuda.synchronize(self.device)

@@ -119,6 +119,7 @@ class JKEngine:
                                             self.pair_off.ctypes.data))
         self.n_elements = ne.value
         self.n_pairs = int(self.pair_off[-1])
+        self.cutoffs = (schwarz, j_density, k_density, pair)
         self._check(lib().tgpu_set_cutoffs(self._ctx, schwarz, j_density, k_density, pair))
         self._check(lib().tgpu_set_fp32_threshold(self._ctx, fp32_threshold))
 
@@ -137,6 +138,11 @@ class JKEngine:
             self.close()
         except Exception:
             pass
+
+    def set_cutoffs_scaled(self, scale):
+        """Schwarz and density cutoffs times `scale` (the pair cutoff is kept)."""
+        s, j, k, p = self.cutoffs
+        self._check(lib().tgpu_set_cutoffs(self._ctx, s * scale, j * scale, k * scale, p))
 
     def set_stream(self, stream_ptr):
         self._check(lib().tgpu_set_stream(self._ctx, ctypes.c_void_p(stream_ptr)))

@@ -33,8 +33,58 @@ def core_guess(h, X, n_occ):
     return 2.0 * Co @ Co.T
 
 
+def atomic_density(h, S, n_electrons, jk, max_iterations=60, tol=1e-9, damp=0.5):
+    """Spherically averaged atomic density for the promolecule guess: restricted SCF with
+    fractional aufbau occupations, the electrons of the open shell spread evenly over its
+    degenerate orbitals (Tonto's promolecule guess, molecule.scf.foo:4827-4844, sums
+    atomic densities)."""
+    X = _sym_orth(S)
+    e, Cp = torch.linalg.eigh(X.T @ h @ X)
+    P = None
+    for it in range(max_iterations):
+        if P is not None:
+            J, K = jk(P, "full")
+            F = h + J - 0.5 * K
+            e, Cp = torch.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        occ = torch.zeros_like(e)
+        left = float(n_electrons)
+        i = 0
+        n = e.shape[0]
+        while left > 1e-12 and i < n:
+            # degenerate set starting at orbital i
+            j = i + 1
+            while j < n and abs(float(e[j] - e[i])) < 1e-4:
+                j += 1
+            cap = 2.0 * (j - i)
+            put = min(cap, left)
+            occ[i:j] = put / (j - i)
+            left -= put
+            i = j
+        P_new = (C * occ) @ C.T
+        if P is not None and damp and it < max_iterations // 2:
+            P_new = damp * P + (1.0 - damp) * P_new
+        if P is not None and (P_new - P).abs().max().item() < tol:
+            return P_new
+        P = P_new
+    return P
+
+
+def promolecule_guess(mol, atom_scf):
+    """Block-diagonal superposition of spherically averaged atomic densities.
+    atom_scf(symbol) -> atomic density matrix (n_bf_atom x n_bf_atom) in the same basis."""
+    cache = {}
+    blocks = []
+    for ia, sym in enumerate(mol.symbols):
+        if sym not in cache:
+            cache[sym] = atom_scf(sym)
+        blocks.append(cache[sym])
+    return torch.block_diag(*blocks)
+
+
 def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e-6,
             max_iterations=100, min_iterations=1, diis_keep=8, diis_start=3,
+            diis_start_error=0.4,
             level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3,
             mixed_until=None, incremental=False, rebuild_every=8, log=None):
     """Run RHF to convergence.
@@ -65,6 +115,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     last_full_fp64 = not low
     force_full = False
     converged = False
+    diis_on = diis_start_error is None
     it = 0
     while True:
         it += 1
@@ -76,7 +127,9 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
             diis_F.pop(0)
             diis_E.pop(0)
         F_use = F
-        if it >= diis_start and len(diis_F) >= 2:
+        if diis_start_error is not None and err_max < diis_start_error:
+            diis_on = True
+        if it >= diis_start and len(diis_F) >= 2 and diis_on:
             m = len(diis_F)
             B = torch.empty((m + 1, m + 1), dtype=dt, device=h.device)
             for i in range(m):
@@ -101,7 +154,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         C = X @ Cp
         Co = C[:, :n_occ]
         P_new = 2.0 * Co @ Co.T
-        if damp and it < damp_iters:
+        if damp and (it < damp_iters or not diis_on):
             P_new = damp * P + (1.0 - damp) * P_new
         full = (not incremental) or force_full or it % rebuild_every == 0
         mode = ("full" if full else "delta") + ("_mixed" if low else "")
