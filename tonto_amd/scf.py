@@ -84,7 +84,7 @@ def promolecule_guess(mol, atom_scf):
 
 def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e-6,
             max_iterations=100, min_iterations=1, diis_keep=8, diis_start=3,
-            diis_start_error=0.4,
+            diis_start_error=None,
             level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3,
             mixed_until=None, incremental=False, rebuild_every=8, log=None):
     """Run RHF to convergence.
