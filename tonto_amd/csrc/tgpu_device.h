@@ -43,7 +43,7 @@ __device__ __forceinline__ void rys_roots(double X, double *u, double *w)
         b1[f] = 0.0;
         b2[f] = 0.0;
     }
-#pragma unroll
+#pragma unroll 2
     for (int m = RYS_NCHEB - 1; m >= 1; --m) {
 #pragma unroll
         for (int f = 0; f < 2 * N; ++f) {
