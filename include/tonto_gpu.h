@@ -42,6 +42,9 @@ enum {
 #define TGPU_JK_MIXED 1    /* quartets with Schwarz bound below the FP32 threshold run in FP32 */
 #define TGPU_JK_GRAPH 2    /* device entry only: capture the build once into a hipGraph and
                               replay it while P/J/K pointers, flags and class forms are unchanged */
+#define TGPU_J_ONLY 4      /* J only, screened by Schwarz_J_test (molecule.fock.foo:3538-3560):
+                              MOLECULE.FOCK:make_r_J_engine (1638-1757); K is returned as zero */
+#define TGPU_K_ONLY 8      /* K only, screened by Schwarz_K_test (3562-): J returned as zero */
 
 typedef struct tgpu_ctx tgpu_ctx;
 

@@ -237,3 +237,23 @@ def test_autotune_keeps_results(torch_cuda):
         e.set_variant(k, 1)
     J2, K2 = e.make_r_JK(g["P"])
     assert np.max(np.abs(J2 - J0)) < 1e-12 and np.max(np.abs(K2 - K0)) < 1e-12
+
+
+@pytest.mark.parametrize("only", ["J", "K"])
+def test_j_only_and_k_only_builds(torch_cuda, oracle_mod, only):
+    """make_r_J_engine (molecule.fock.foo:1638-1757, Schwarz_J_test 3538-3560) and the K-only
+    build (Schwarz_K_test): the density test has a 1e-20 floor, the other matrix is absent."""
+    from tonto_amd.geometry import water_cluster
+    g = np.load(os.path.join(HERE, "golden", "w4_6311gdp.npz"))
+    mol = water_cluster(4, "6-311G(d,p)", seed=int(g["seed"]))
+    e = _engine(mol)
+    mi = e.schwarz()
+    J, K = e.make_r_JK(g["P"], only=only)
+    o = oracle_mod.OracleSystem(mol)
+    jp, kp = (1e-20, float("inf")) if only == "J" else (float("inf"), 1e-20)
+    Jo, Ko, so = o.make_r_JK(g["P"], jp=jp, kp=kp, max_I=mi, return_stats=True)
+    assert e.stats()["quartets"] == so["quartets"]
+    if only == "J":
+        assert not K.any() and np.max(np.abs(J - Jo)) <= 1e-10
+    else:
+        assert not J.any() and np.max(np.abs(K - Ko)) <= 1e-10

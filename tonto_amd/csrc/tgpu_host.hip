@@ -760,9 +760,10 @@ static const int kClassPair[NQCLS][2] = {
 static int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
 
 // Enqueue one complete build on stream s (no host synchronisation).
-static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, bool mixed,
+static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, int flags,
                           hipStream_t s, bool timing)
 {
+    const bool mixed = flags & TGPU_JK_MIXED;
     const BatchArgs ba = batch_args(c);
     fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_X.p, c->n_elem);
     fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_Y.p, c->n_elem);
@@ -780,6 +781,10 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, bo
     a.cutoff = c->cut_schwarz;
     a.jp = c->cut_jp;
     a.kp = c->cut_kp;
+    // J-only / K-only builds: Schwarz_J_test / Schwarz_K_test have a TOL(20) density floor
+    // in place of the density cutoff, and the other matrix is never digested
+    if (flags & TGPU_J_ONLY) { a.jp = 1e-20; a.kp = HUGE_VAL; }
+    if (flags & TGPU_K_ONLY) { a.kp = 1e-20; a.jp = HUGE_VAL; }
     a.fp32_thr = c->fp32_thr;
     a.X = c->d_X.p;
     a.Y = c->d_Y.p;
@@ -815,9 +820,13 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
         int rc = tgpu_schwarz(c, nullptr);
         if (rc) return rc;
     }
-    const bool mixed = flags & TGPU_JK_MIXED;
+    if ((flags & TGPU_J_ONLY) && (flags & TGPU_K_ONLY)) {
+        c->err = "TGPU_J_ONLY and TGPU_K_ONLY are exclusive";
+        return TGPU_E_ARG;
+    }
+    const int kflags = flags & (TGPU_JK_MIXED | TGPU_J_ONLY | TGPU_K_ONLY);
     if (!(flags & TGPU_JK_GRAPH)) {
-        enqueue_build(c, P, J, K, mixed, c->stream, true);
+        enqueue_build(c, P, J, K, kflags, c->stream, true);
         c->timed = true;
         HIPCHK(c, hipGetLastError());
         return TGPU_OK;
@@ -825,11 +834,11 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
     // captured build: (re)capture when the pointers, flags or class variants changed
     const std::vector<int> var(c->variant, c->variant + NQCLS);
     if (!c->gexec || c->g_key[0] != P || c->g_key[1] != J || c->g_key[2] != K ||
-        c->g_flags != (flags & TGPU_JK_MIXED) || c->g_variant != var) {
+        c->g_flags != kflags || c->g_variant != var) {
         drop_graph(c);
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
-        enqueue_build(c, P, J, K, mixed, c->gstream, false);
+        enqueue_build(c, P, J, K, kflags, c->gstream, false);
         HIPCHK(c, hipStreamEndCapture(c->gstream, &g));
         hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -839,7 +848,7 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
             return TGPU_E_HIP;
         }
         c->g_key[0] = P; c->g_key[1] = J; c->g_key[2] = K;
-        c->g_flags = flags & TGPU_JK_MIXED;
+        c->g_flags = kflags;
         c->g_variant = var;
     }
     HIPCHK(c, hipEventRecord(c->ev_in, c->stream));

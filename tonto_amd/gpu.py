@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "libtonto_gpu.so")
 
 TGPU_JK_MIXED = 1
 TGPU_JK_GRAPH = 2
+TGPU_J_ONLY = 4
+TGPU_K_ONLY = 8
 
 # exported symbols, in the order of include/tonto_gpu.h (plus the two extensions)
 EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_set_stream",
@@ -159,15 +161,15 @@ class JKEngine:
     def unpack(self, flat):
         return [flat[o:o + n * n].reshape(n, n) for o, n in zip(self.mat_off, self.nbf)]
 
-    def make_r_JK(self, P, mixed=False):
+    def make_r_JK(self, P, mixed=False, only=None):
         """Host arrays in, host arrays out (synchronous).  For a batch, P is a list of
         matrices (or the packed vector) and lists are returned."""
         batch = self.n_mol > 1 or isinstance(P, (list, tuple))
         Pf = self.pack(P) if isinstance(P, (list, tuple)) else np.ascontiguousarray(P, dtype=np.float64).ravel()
         assert Pf.size == self.n_elements
         J, K = np.zeros(self.n_elements), np.zeros(self.n_elements)
-        self._check(lib().tgpu_make_r_JK(self._ctx, _dp(Pf), _dp(J), _dp(K),
-                                         TGPU_JK_MIXED if mixed else 0))
+        flags = (TGPU_JK_MIXED if mixed else 0) | {None: 0, "J": TGPU_J_ONLY, "K": TGPU_K_ONLY}[only]
+        self._check(lib().tgpu_make_r_JK(self._ctx, _dp(Pf), _dp(J), _dp(K), flags))
         if batch:
             return self.unpack(J), self.unpack(K)
         n = self.n_bf
