@@ -112,8 +112,14 @@ __device__ __forceinline__ long long tonto_pair(int x, int y)
     return x >= y ? (long long)x * (x + 1) / 2 + y : (long long)y * (y + 1) / 2 + x;
 }
 
-// Per-molecule layout of a batch: molecule m's pairs of class c occupy
-// [seg_start[m*NCLS+c], +seg_count) of the sorted pair arrays; its matrices start at
+// Pair bins: every angular-momentum class is split by the primitive count of the pair
+// (full list) into TGPU_NBUCKET buckets, so that the lanes of a wavefront run primitive
+// loops of similar length; bin = class * TGPU_NBUCKET + bucket.
+#define TGPU_NBUCKET 4
+#define TGPU_NBIN (TGPU_NPAIRCLASS * TGPU_NBUCKET)
+
+// Per-molecule layout of a batch: molecule m's pairs of bin k occupy
+// [seg_start[m*NBIN+k], +seg_count) of the sorted pair arrays; its matrices start at
 // element mat_off[m] with leading dimension nbf[m]; its Tonto pairs start at pair_off[m]
 // in the P_max array; scal[2m], scal[2m+1] are its P_max and IP_max.
 struct DevMols {
@@ -125,7 +131,7 @@ struct JKArgs {
     DevPairs pr;
     DevPrims pm;
     DevMols ml;
-    int bra_start, n_bra, bra_cls, ket_cls, same_bin, counter_slot;
+    int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot;
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -144,9 +150,9 @@ __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
 {
     b.sb = a.bra_start + i;
     const int m = a.pr.mol[b.sb];
-    b.kb = a.ml.seg_start[m * TGPU_NPAIRCLASS + a.ket_cls];
-    const int kn = a.ml.seg_count[m * TGPU_NPAIRCLASS + a.ket_cls];
-    const int jlim = a.same_bin ? b.sb - a.ml.seg_start[m * TGPU_NPAIRCLASS + a.bra_cls] + 1 : kn;
+    b.kb = a.ml.seg_start[m * TGPU_NBIN + a.ket_bin];
+    const int kn = a.ml.seg_count[m * TGPU_NBIN + a.ket_bin];
+    const int jlim = a.same_bin ? b.sb - a.ml.seg_start[m * TGPU_NBIN + a.bra_bin] + 1 : kn;
     const long long mo = a.ml.mat_off[m];
     b.ld = a.ml.nbf[m];
     b.P = a.P + mo;

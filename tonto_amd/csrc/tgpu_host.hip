@@ -30,6 +30,14 @@
 
 static constexpr int NCLS = TGPU_NPAIRCLASS;
 static constexpr int NQCLS = NCLS * (NCLS + 1) / 2;
+static constexpr int NBK = TGPU_NBUCKET;
+static constexpr int NBIN = TGPU_NBIN;
+
+// primitive-count bucket of a shell pair (full primitive list)
+static int prim_bucket(int n)
+{
+    return n <= 1 ? 0 : n <= 4 ? 1 : n <= 12 ? 2 : 3;
+}
 
 // ----------------------------------------------------------------------------------
 // Small kernels
@@ -278,9 +286,10 @@ struct tgpu_ctx {
     std::vector<double> xyz, Z, exps, coefs;
     std::vector<int> sh_l, sh_atom, sh_np, sh_off, sh_bf, sh_mol;
     double cut_schwarz = 1e-9, cut_jp = 1e-9, cut_kp = 1e-9, cut_pair = 1e-6, fp32_thr = 1e-6;
-    // pair tables in device order: class bin, molecule, (after Schwarz) max_I descending
-    int bin_start[NCLS + 1] = {0};
-    std::vector<int> seg_start, seg_count;  // [n_mol * NCLS]
+    // pair tables in device order: bin (class, primitive bucket), molecule, (after
+    // Schwarz) max_I descending
+    int bin_start[NBIN + 1] = {0};
+    std::vector<int> seg_start, seg_count;  // [n_mol * NBIN]
     std::vector<int> tonto, sp, sq, bfp, bfq, atp, atq, pmol, off_full, n_full, off_skip, n_skip;
     std::vector<double> A, AB, maxI;
     std::vector<double> pz, pK, pPx, pPy, pPz;
@@ -545,29 +554,31 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
         c->err = "too many shell pairs for 32-bit pair indexing";
         return TGPU_E_UNSUPPORTED;
     }
-    // pairs: class bin, then molecule, then Tonto order
-    std::vector<std::vector<std::pair<int, int>>> bins(NCLS);  // (molecule, local Tonto index)
+    // pairs: bin (class, primitive bucket), then molecule, then Tonto order
+    std::vector<std::vector<std::pair<int, int>>> bins(NBIN);  // (molecule, local Tonto index)
     for (int m = 0; m < n_mol; ++m) {
         const int s0 = c->shell_off[m], ns = c->shell_off[m + 1] - s0;
         for (int a = 0; a < ns; ++a)
             for (int b = 0; b <= a; ++b) {
                 const int la = c->sh_l[s0 + a], lb = c->sh_l[s0 + b];
-                bins[pair_class(std::max(la, lb), std::min(la, lb))].push_back({m, a * (a + 1) / 2 + b});
+                const int bin = pair_class(std::max(la, lb), std::min(la, lb)) * NBK +
+                                prim_bucket(c->sh_np[s0 + a] * c->sh_np[s0 + b]);
+                bins[bin].push_back({m, a * (a + 1) / 2 + b});
             }
     }
-    c->seg_start.assign((size_t)n_mol * NCLS, 0);
-    c->seg_count.assign((size_t)n_mol * NCLS, 0);
-    for (int k = 0; k < NCLS; ++k) {
+    c->seg_start.assign((size_t)n_mol * NBIN, 0);
+    c->seg_count.assign((size_t)n_mol * NBIN, 0);
+    for (int k = 0; k < NBIN; ++k) {
         c->bin_start[k] = (int)c->tonto.size();
         for (auto &pr : bins[k]) {
-            if (c->seg_count[(size_t)pr.first * NCLS + k] == 0)
-                c->seg_start[(size_t)pr.first * NCLS + k] = (int)c->tonto.size();
-            c->seg_count[(size_t)pr.first * NCLS + k]++;
+            if (c->seg_count[(size_t)pr.first * NBIN + k] == 0)
+                c->seg_start[(size_t)pr.first * NBIN + k] = (int)c->tonto.size();
+            c->seg_count[(size_t)pr.first * NBIN + k]++;
             c->pmol.push_back(pr.first);
             c->tonto.push_back(pr.second);
         }
     }
-    c->bin_start[NCLS] = (int)c->tonto.size();
+    c->bin_start[NBIN] = (int)c->tonto.size();
     const size_t np = c->tonto.size();
     c->sp.resize(np); c->sq.resize(np); c->bfp.resize(np); c->bfq.resize(np);
     c->atp.resize(np); c->atq.resize(np); c->A.resize(3 * np); c->AB.resize(3 * np);
@@ -707,8 +718,8 @@ extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
         const DevPrims dpm = dev_prims(c);
 #define TGPU_SCHWARZ_CASE(LP, LQ, H)                                                       \
     {                                                                                    \
-        const int cls = pair_class(LP, LQ), n = c->bin_start[cls + 1] - c->bin_start[cls]; \
-        if (n) tgpu_launch_schwarz_##LP##LQ(dpr, dpm, c->bin_start[cls], n, c->d_maxI.p, c->stream); \
+        const int b0 = pair_class(LP, LQ) * NBK, n = c->bin_start[b0 + NBK] - c->bin_start[b0]; \
+        if (n) tgpu_launch_schwarz_##LP##LQ(dpr, dpm, c->bin_start[b0], n, c->d_maxI.p, c->stream); \
     }
         TGPU_FOR_EACH_PAIRCLASS(TGPU_SCHWARZ_CASE)
 #undef TGPU_SCHWARZ_CASE
@@ -790,19 +801,26 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.Y = c->d_Y.p;
     a.counters = c->d_counters.p;
     for (int k = 0; k < NQCLS; ++k) {
-        const int X = kClassPair[k][0], Y = kClassPair[k][1];
-        a.bra_cls = X;
-        a.ket_cls = Y;
-        a.bra_start = c->bin_start[X];
-        a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
-        a.same_bin = X == Y;
-        a.counter_slot = k;
-        const int n_ket = c->bin_start[Y + 1] - c->bin_start[Y];
+        const int CX = kClassPair[k][0], CY = kClassPair[k][1];
         const bool prof = timing && c->profiling;
         if (prof) c->cls_launched[k] = 0;
-        if (a.n_bra == 0 || n_ket == 0) continue;
+        const int nb = c->bin_start[CX * NBK + NBK] - c->bin_start[CX * NBK];
+        const int nk = c->bin_start[CY * NBK + NBK] - c->bin_start[CY * NBK];
+        if (nb == 0 || nk == 0) continue;
         if (prof) (void)hipEventRecord(c->cls_ev[2 * k], s);
-        kLaunch[k](a, c->variant[k], mixed, s);
+        a.counter_slot = k;
+        // every unordered pair of bins once: within one class, ket bucket <= bra bucket
+        for (int bx = 0; bx < NBK; ++bx)
+            for (int by = 0; by < (CX == CY ? bx + 1 : NBK); ++by) {
+                const int X = CX * NBK + bx, Y = CY * NBK + by;
+                a.bra_bin = X;
+                a.ket_bin = Y;
+                a.bra_start = c->bin_start[X];
+                a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
+                a.same_bin = X == Y;
+                if (a.n_bra == 0 || c->bin_start[Y + 1] == c->bin_start[Y]) continue;
+                kLaunch[k](a, c->variant[k], mixed, s);
+            }
         if (prof) {
             (void)hipEventRecord(c->cls_ev[2 * k + 1], s);
             c->cls_launched[k] = 1;
