@@ -92,26 +92,27 @@ def gen_class(LP, LQ, LR, LS, R):
     em = Emitter()
     em(f"template <> struct EriBody<{LP}, {LQ}, {LR}, {LS}, {R}> {{", 0)
     em("static constexpr int NP = %d, NQ = %d, NR = %d, NS = %d, NROOT = %d;" % (NP, NQ, NR, NS, nroot))
-    em("static __device__ __forceinline__ void run(const PairView &bra, const PairView &ket, double *__restrict__ out)")
+    em("static __device__ __forceinline__ void run(const PairView &bra, const PairView &ket, double *__restrict__ out,")
+    em("                                           const double *__restrict__ rys_tab)")
     em("{")
     em(f"{R} G[{NE * NF}];", 2)
     em(f"#pragma unroll", 2)
     em(f"for (int k = 0; k < {NE * NF}; ++k) G[k] = ({R})0;", 2)
     em("for (int ip = 0; ip < bra.n; ++ip) {", 2)
-    em("const double za = bra.zeta[ip], Ka = bra.K[ip];", 3)
+    em("const double za = bra.zeta[ip], iza = bra.zinv[ip], Ka = bra.K[ip];", 3)
     em("const double Px = bra.Px[ip], Py = bra.Py[ip], Pz = bra.Pz[ip];", 3)
     em(f"const {R} PAx = ({R})(Px - bra.Ax), PAy = ({R})(Py - bra.Ay), PAz = ({R})(Pz - bra.Az);", 3)
     em("for (int iq = 0; iq < ket.n; ++iq) {", 3)
-    em("const double zc = ket.zeta[iq], Kc = ket.K[iq];", 4)
+    em("const double zc = ket.zeta[iq], izc = ket.zinv[iq], Kc = ket.K[iq];", 4)
     em("const double Qx = ket.Px[iq], Qy = ket.Py[iq], Qz = ket.Pz[iq];", 4)
-    em("const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;", 4)
+    em("const double izs = 1.0 / (za + zc), rho = za * zc * izs;", 4)
     em("const double QPx = Qx - Px, QPy = Qy - Py, QPz = Qz - Pz;", 4)
     em("const double X = rho * (QPx * QPx + QPy * QPy + QPz * QPz);", 4)
     em(f"double u[{nroot}], w[{nroot}];", 4)
-    em(f"rys_roots<{nroot}>(X, u, w);", 4)
+    em(f"rys_roots_tab<{nroot}>(X, u, w, rys_tab);", 4)
     em("const double pref = TWOPI5ON2 * Ka * Kc * sqrt(rho);", 4)
-    em("const double rz0 = rho / za, re0 = rho / zc;", 4)
-    em(f"const {R} hz = ({R})(0.5 / za), he = ({R})(0.5 / zc), hs = ({R})(0.5 * izs);", 4)
+    em("const double rz0 = zc * izs, re0 = za * izs;   // rho/zeta, rho/eta", 4)
+    em(f"const {R} hz = ({R})(0.5 * iza), he = ({R})(0.5 * izc), hs = ({R})(0.5 * izs);", 4)
     em(f"const {R} QCx = ({R})(Qx - ket.Ax), QCy = ({R})(Qy - ket.Ay), QCz = ({R})(Qz - ket.Az);", 4)
     em(f"const {R} qpx = ({R})QPx, qpy = ({R})QPy, qpz = ({R})QPz;", 4)
     em(f"#pragma unroll", 4)

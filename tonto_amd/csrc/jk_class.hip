@@ -27,7 +27,10 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
 #endif
     (void)variant;
     (void)mixed;  // the wavefront form runs in FP64
-    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<a.n_bra, 64, 0, s>>>(a);
+    // one work-group per (bra pair, chunk of a.ket_chunk kets); chunks past a bra pair's
+    // Schwarz frontier exit at once
+    const int chunks = (a.ket_chunk > 0 ? (a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk : 1);
+    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3(a.n_bra, chunks > 0 ? chunks : 1), 64, 0, s>>>(a);
 }
 
 #if TGPU_LP == TGPU_LR && TGPU_LQ == TGPU_LS

@@ -6,7 +6,7 @@
 #include "../../include/tonto_gpu.h"
 #include "gen/classes.inc"
 
-#define RYS_QUAL static __device__ const
+#define RYS_QUAL static __device__ const __attribute__((aligned(16)))
 #include "rys_table.inc"
 
 #define TWOPI5ON2 34.9868366552497256925  // include/macros.in:446
@@ -15,7 +15,7 @@
 // Device helpers
 // ----------------------------------------------------------------------------------
 struct PairView {
-    const double *zeta, *K, *Px, *Py, *Pz;
+    const double *zeta, *zinv, *K, *Px, *Py, *Pz;
     int n;
     double Ax, Ay, Az, ABx, ABy, ABz;
 };
@@ -59,6 +59,61 @@ __device__ __forceinline__ void rys_roots(double X, double *u, double *w)
     }
 }
 
+// Same evaluation with the n-root coefficient block passed in (LDS copy or global):
+// tab points at rys_table_data + rys_table_offset[N]; pairs (root, weight) are read as
+// 16-byte vectors.
+template <int N>
+__device__ __forceinline__ void rys_roots_tab(double X, double *u, double *w, const double *__restrict__ tab)
+{
+    if (X >= rys_xasym[N]) {
+        const double ix = 1.0 / X, isx = sqrt(ix);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            u[i] = rys_herm_s2[(N - 1) * RYS_NMAX + i] * ix;
+            w[i] = rys_herm_w[(N - 1) * RYS_NMAX + i] * isx;
+        }
+        return;
+    }
+    const int k = (int)(X * (1.0 / RYS_H));
+    const double t = (X - k * RYS_H) * (2.0 / RYS_H) - 1.0, tt = 2.0 * t;
+    const double2 *c = reinterpret_cast<const double2 *>(tab + k * (RYS_NCHEB * 2 * N));
+    double b1[2 * N], b2[2 * N];
+#pragma unroll
+    for (int f = 0; f < 2 * N; ++f) {
+        b1[f] = 0.0;
+        b2[f] = 0.0;
+    }
+#pragma unroll 2
+    for (int m = RYS_NCHEB - 1; m >= 1; --m) {
+#pragma unroll
+        for (int h = 0; h < N; ++h) {
+            const double2 cm = c[m * N + h];
+            const double b0x = tt * b1[2 * h] - b2[2 * h] + cm.x;
+            const double b0y = tt * b1[2 * h + 1] - b2[2 * h + 1] + cm.y;
+            b2[2 * h] = b1[2 * h];
+            b1[2 * h] = b0x;
+            b2[2 * h + 1] = b1[2 * h + 1];
+            b1[2 * h + 1] = b0y;
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < N; ++h) {
+        const double2 c0 = c[h];
+        const double v0 = t * b1[2 * h] - b2[2 * h] + c0.x;
+        const double v1 = t * b1[2 * h + 1] - b2[2 * h + 1] + c0.y;
+        // the 2N functions are stored roots first, then weights
+        if (2 * h < N) u[2 * h] = v0; else w[2 * h - N] = v0;
+        if (2 * h + 1 < N) u[2 * h + 1] = v1; else w[2 * h + 1 - N] = v1;
+    }
+}
+
+// number of doubles in the n-root block of the table
+__host__ __device__ constexpr int rys_block_size(int n)
+{
+    return (n == 1 ? 20 : n == 2 ? 24 : n == 3 ? 27 : n == 4 ? 31 : n == 5 ? 34 : n == 6 ? 37 : 39) *
+           RYS_NCHEB * 2 * n;
+}
+
 static __device__ void rys_roots_rt(int n, double X, double *u, double *w)
 {
     switch (n) {
@@ -84,7 +139,7 @@ struct DevPairs {
     const double *A, *AB, *maxI;
 };
 struct DevPrims {
-    const double *zeta, *K, *Px, *Py, *Pz;
+    const double *zeta, *zinv, *K, *Px, *Py, *Pz;
 };
 
 __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims &pm, int s,
@@ -94,6 +149,7 @@ __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims
     const int off = skip ? pr.off_skip[s] : pr.off_full[s];
     v.n = skip ? pr.n_skip[s] : pr.n_full[s];
     v.zeta = pm.zeta + off;
+    v.zinv = pm.zinv + off;
     v.K = pm.K + off;
     v.Px = pm.Px + off;
     v.Py = pm.Py + off;
@@ -131,7 +187,7 @@ struct JKArgs {
     DevPairs pr;
     DevPrims pm;
     DevMols ml;
-    int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot;
+    int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot, ket_chunk, n_ket_max;
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -226,6 +282,13 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
 {
     constexpr int NP = ncart(LP), NQ = ncart(LQ), NR = ncart(LR), NS = ncart(LS);
     constexpr int NPQ = NP * NQ, NRS = NR * NS, NI = NPQ * NRS;
+    constexpr int NRT = EriBody<LP, LQ, LR, LS, double>::NROOT;
+    // the n-root Rys table in LDS (<= 18 KB): the root/weight lookups of every primitive
+    // quartet then hit LDS instead of L2
+    __shared__ __attribute__((aligned(16))) double rys_lds[rys_block_size(NRT)];
+    for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
+        rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= a.n_bra) return;  // wave-uniform
@@ -260,9 +323,9 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         double I[NI];
         if (!MIXED || j < jsplit) {
-            EriBody<LP, LQ, LR, LS, double>::run(bv, kv, I);
+            EriBody<LP, LQ, LR, LS, double>::run(bv, kv, I, rys_lds);
         } else {
-            EriBody<LP, LQ, LR, LS, float>::run(bv, kv, I);
+            EriBody<LP, LQ, LR, LS, float>::run(bv, kv, I, rys_lds);
             ++n32;
         }
         ++nq;
@@ -392,7 +455,7 @@ __global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, i
     const int s = start + i;
     const PairView v = make_view(pr, pm, s, false);
     double I[NP * NQ * NP * NQ];
-    EriBody<LP, LQ, LP, LQ, double>::run(v, v, I);
+    EriBody<LP, LQ, LP, LQ, double>::run(v, v, I, rys_table_data + rys_table_offset[EriBody<LP, LQ, LP, LQ, double>::NROOT]);
     double m = 0.0;
 #pragma unroll
     for (int p = 0; p < NP; ++p)
@@ -536,6 +599,10 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
     if ((int)blockIdx.x >= a.n_bra) return;
     BraInfo b;
     load_bra(a, blockIdx.x, b);
+    // this work-group's chunk of the ket prefix
+    const int j_begin = blockIdx.y * a.ket_chunk;
+    const int j_end = min(b.jend, j_begin + a.ket_chunk);
+    if (j_begin >= j_end) return;
     const int ld = b.ld, fp = b.fp, fq = b.fq;
     const double *__restrict__ P = b.P;
     for (int t = lane; t < NPQ; t += 64) L.Ppq[t] = P[(size_t)(fp + t / NQ) * ld + fq + t % NQ];
@@ -543,7 +610,7 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
 #pragma unroll
     for (int m = 0; m < MJ; ++m) Jpq[m] = 0.0;
     unsigned long long nq = 0, npq = 0;
-    for (int j = 0; j < b.jend; ++j) {  // uniform over the work-group
+    for (int j = j_begin; j < j_end; ++j) {  // uniform over the work-group
         KetTest t;
         if (!test_ket(a, b, j, t)) continue;
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);

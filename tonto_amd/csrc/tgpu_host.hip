@@ -292,7 +292,8 @@ struct tgpu_ctx {
     std::vector<int> seg_start, seg_count;  // [n_mol * NBIN]
     std::vector<int> tonto, sp, sq, bfp, bfq, atp, atq, pmol, off_full, n_full, off_skip, n_skip;
     std::vector<double> A, AB, maxI;
-    std::vector<double> pz, pK, pPx, pPy, pPz;
+    std::vector<double> pz, pzi, pK, pPx, pPy, pPz;
+    std::vector<int> bin_max_seg;  // [NBIN] largest molecule segment of each bin
     bool have_schwarz = false;
     std::vector<double> maxI_mol, maxI_tonto;
     int variant[NQCLS];
@@ -301,7 +302,7 @@ struct tgpu_ctx {
         d_off_skip, d_n_skip, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
         d_atom_off;
     DBuf<long long> d_pair_off, d_mat_off;
-    DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol, d_pz, d_pK, d_pPx, d_pPy, d_pPz;
+    DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol, d_pz, d_pzi, d_pK, d_pPx, d_pPy, d_pPz;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
     DBuf<unsigned long long> d_counters;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
@@ -367,7 +368,7 @@ static void drop_graph(tgpu_ctx *c)
 // (shell1quartet.foo:201-270) so that the skip decisions are bit-identical.
 static void build_prims(tgpu_ctx *c)
 {
-    c->pz.clear(); c->pK.clear(); c->pPx.clear(); c->pPy.clear(); c->pPz.clear();
+    c->pz.clear(); c->pzi.clear(); c->pK.clear(); c->pPx.clear(); c->pPy.clear(); c->pPz.clear();
     const size_t np = c->tonto.size();
     c->off_full.assign(np, 0); c->n_full.assign(np, 0);
     c->off_skip.assign(np, 0); c->n_skip.assign(np, 0);
@@ -404,6 +405,7 @@ static void build_prims(tgpu_ctx *c)
                         for (int t = 0; t < 3; ++t) P[t] = (be * PB[t] + ae * PA[t]) * inv;
                     }
                     c->pz.push_back(sum);
+                    c->pzi.push_back(inv);
                     c->pK.push_back(pref);
                     c->pPx.push_back(P[0]);
                     c->pPy.push_back(P[1]);
@@ -428,7 +430,8 @@ static int upload_pairs(tgpu_ctx *c)
         (rc = dupload(c, c->d_off_full, c->off_full)) || (rc = dupload(c, c->d_n_full, c->n_full)) ||
         (rc = dupload(c, c->d_off_skip, c->off_skip)) || (rc = dupload(c, c->d_n_skip, c->n_skip)) ||
         (rc = dupload(c, c->d_A, c->A)) || (rc = dupload(c, c->d_AB, c->AB)) ||
-        (rc = dupload(c, c->d_pz, c->pz)) || (rc = dupload(c, c->d_pK, c->pK)) ||
+        (rc = dupload(c, c->d_pz, c->pz)) || (rc = dupload(c, c->d_pzi, c->pzi)) ||
+        (rc = dupload(c, c->d_pK, c->pK)) ||
         (rc = dupload(c, c->d_pPx, c->pPx)) || (rc = dupload(c, c->d_pPy, c->pPy)) ||
         (rc = dupload(c, c->d_pPz, c->pPz)) || (rc = dupload(c, c->d_seg_start, c->seg_start)) ||
         (rc = dupload(c, c->d_seg_count, c->seg_count)))
@@ -452,7 +455,7 @@ static DevPairs dev_pairs(const tgpu_ctx *c)
 static DevPrims dev_prims(const tgpu_ctx *c)
 {
     DevPrims d;
-    d.zeta = c->d_pz.p; d.K = c->d_pK.p; d.Px = c->d_pPx.p; d.Py = c->d_pPy.p; d.Pz = c->d_pPz.p;
+    d.zeta = c->d_pz.p; d.zinv = c->d_pzi.p; d.K = c->d_pK.p; d.Px = c->d_pPx.p; d.Py = c->d_pPy.p; d.Pz = c->d_pPz.p;
     return d;
 }
 
@@ -579,6 +582,10 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
         }
     }
     c->bin_start[NBIN] = (int)c->tonto.size();
+    c->bin_max_seg.assign(NBIN, 0);
+    for (int m = 0; m < n_mol; ++m)
+        for (int k = 0; k < NBIN; ++k)
+            c->bin_max_seg[k] = std::max(c->bin_max_seg[k], c->seg_count[(size_t)m * NBIN + k]);
     const size_t np = c->tonto.size();
     c->sp.resize(np); c->sq.resize(np); c->bfp.resize(np); c->bfq.resize(np);
     c->atp.resize(np); c->atq.resize(np); c->A.resize(3 * np); c->AB.resize(3 * np);
@@ -633,7 +640,7 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_off_skip); dfree(c->d_n_skip); dfree(c->d_sh_l); dfree(c->d_sh_bf);
     dfree(c->d_seg_start); dfree(c->d_seg_count); dfree(c->d_nbf); dfree(c->d_shell_off);
     dfree(c->d_atom_off); dfree(c->d_pair_off); dfree(c->d_mat_off);
-    dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_pz);
+    dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_pz); dfree(c->d_pzi);
     dfree(c->d_pK); dfree(c->d_pPx); dfree(c->d_pPy); dfree(c->d_pPz); dfree(c->d_X); dfree(c->d_Y);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
     dfree(c->d_counters);
@@ -818,6 +825,8 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
                 a.bra_start = c->bin_start[X];
                 a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
                 a.same_bin = X == Y;
+                a.n_ket_max = c->bin_max_seg[Y];
+                a.ket_chunk = 16;
                 if (a.n_bra == 0 || c->bin_start[Y + 1] == c->bin_start[Y]) continue;
                 kLaunch[k](a, c->variant[k], mixed, s);
             }
