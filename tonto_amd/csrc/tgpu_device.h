@@ -133,11 +133,16 @@ static constexpr int pair_class(int lp, int lq) { return lp * (lp + 1) / 2 + lq;
 template <int LP, int LQ, int LR, int LS, typename R> struct EriBody;
 template <int LP, int LQ, int LR, int LS> struct WaveTab;
 
+// Per-pair integer fields packed into two 16-byte records so that a lane fetches a
+// ket pair's description with two vector loads (coalesced across the wavefront):
+//   rec0 = {Tonto index, shells p<<16|q, atoms p<<16|q, first bf p<<16|q}
+//   rec1 = {full-list offset, skipped-list offset, counts full<<16|skipped, molecule}
 struct DevPairs {
-    const int *tonto, *sh_p, *sh_q, *bf_p, *bf_q, *atom_p, *atom_q, *mol;
-    const int *off_full, *n_full, *off_skip, *n_skip;
+    const int4 *rec0, *rec1;
     const double *A, *AB, *maxI;
 };
+__device__ __forceinline__ int hi16(int v) { return (int)((unsigned)v >> 16); }
+__device__ __forceinline__ int lo16(int v) { return v & 0xffff; }
 struct DevPrims {
     const double *zeta, *zinv, *K, *Px, *Py, *Pz;
 };
@@ -146,8 +151,9 @@ __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims
                                               bool skip)
 {
     PairView v;
-    const int off = skip ? pr.off_skip[s] : pr.off_full[s];
-    v.n = skip ? pr.n_skip[s] : pr.n_full[s];
+    const int4 r1 = pr.rec1[s];
+    const int off = skip ? r1.y : r1.x;
+    v.n = skip ? lo16(r1.z) : hi16(r1.z);
     v.zeta = pm.zeta + off;
     v.zinv = pm.zinv + off;
     v.K = pm.K + off;
@@ -205,7 +211,8 @@ struct BraInfo {
 __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
 {
     b.sb = a.bra_start + i;
-    const int m = a.pr.mol[b.sb];
+    const int4 r0 = a.pr.rec0[b.sb];
+    const int m = a.pr.rec1[b.sb].w;
     b.kb = a.ml.seg_start[m * TGPU_NBIN + a.ket_bin];
     const int kn = a.ml.seg_count[m * TGPU_NBIN + a.ket_bin];
     const int jlim = a.same_bin ? b.sb - a.ml.seg_start[m * TGPU_NBIN + a.bra_bin] + 1 : kn;
@@ -217,14 +224,14 @@ __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
     b.Pmax = a.Pmax + a.ml.pair_off[m];
     const double P_max = a.scal[2 * m];
     b.IP_max = a.scal[2 * m + 1];
-    b.tb = a.pr.tonto[b.sb];
+    b.tb = r0.x;
     b.mIb = a.pr.maxI[b.sb];
-    b.bp = a.pr.sh_p[b.sb];
-    b.bq = a.pr.sh_q[b.sb];
-    b.atp = a.pr.atom_p[b.sb];
-    b.atq = a.pr.atom_q[b.sb];
-    b.fp = a.pr.bf_p[b.sb];
-    b.fq = a.pr.bf_q[b.sb];
+    b.bp = hi16(r0.y);
+    b.bq = lo16(r0.y);
+    b.atp = hi16(r0.z);
+    b.atq = lo16(r0.z);
+    b.fp = hi16(r0.w);
+    b.fq = lo16(r0.w);
     b.Pmax_b = b.Pmax[b.tb];
     b.maxI = a.pr.maxI + b.kb;
     // kets are sorted by max_I descending, so those passing
@@ -239,7 +246,7 @@ __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
 }
 
 struct KetTest {
-    int sk;
+    int sk, fr, fs;
     bool skipJ, skipK, skip_lists;
     double f;
 };
@@ -250,12 +257,13 @@ struct KetTest {
 __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int j, KetTest &t)
 {
     t.sk = b.kb + j;
-    const int tk = a.pr.tonto[t.sk];
+    const int4 r0 = a.pr.rec0[t.sk];
+    const int tk = r0.x;
     const double mIk = b.maxI[j];
     const bool bhi = b.tb >= tk;  // Tonto's (ab| is the pair with the larger index
     const double Iab = bhi ? b.mIb : mIk, Icd = bhi ? mIk : b.mIb;
     if (Iab * b.IP_max < a.cutoff) return false;
-    const int kr = a.pr.sh_p[t.sk], ks = a.pr.sh_q[t.sk];
+    const int kr = hi16(r0.y), ks = lo16(r0.y);
     const double Pmax_k = b.Pmax[tk];
     const double JP = bhi ? fmax(b.Pmax_b, Pmax_k) : fmax(Pmax_k, b.Pmax_b);
     const double KP = fmax(fmax(b.Pmax[tonto_pair(b.bp, kr)], b.Pmax[tonto_pair(b.bp, ks)]),
@@ -267,7 +275,9 @@ __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int 
     if (kr == ks) f *= 0.5;
     if (b.sb == t.sk) f *= 0.5;
     t.f = f;
-    const int akr = a.pr.atom_p[t.sk], aks = a.pr.atom_q[t.sk];
+    const int akr = hi16(r0.z), aks = lo16(r0.z);
+    t.fr = hi16(r0.w);
+    t.fs = lo16(r0.w);
     const bool any_same = b.atp == b.atq || akr == aks || b.atp == akr || b.atp == aks ||
                           b.atq == akr || b.atq == aks;
     t.skip_lists = !any_same;
@@ -331,7 +341,7 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
         const double f = t.f;
-        const int fr = a.pr.bf_p[t.sk], fs = a.pr.bf_q[t.sk];
+        const int fr = t.fr, fs = t.fs;
         if (!t.skipJ) {
             double Prs[NRS];
 #pragma unroll
@@ -616,7 +626,7 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         const double f = t.f;
-        const int fr = a.pr.bf_p[t.sk], fs = a.pr.bf_q[t.sk];
+        const int fr = t.fr, fs = t.fs;
         // stage the ket-dependent density blocks
         for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
         for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fs + u / NQ) * ld + fq + u % NQ];

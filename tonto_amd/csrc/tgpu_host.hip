@@ -298,8 +298,7 @@ struct tgpu_ctx {
     std::vector<double> maxI_mol, maxI_tonto;
     int variant[NQCLS];
     // device
-    DBuf<int> d_tonto, d_sp, d_sq, d_bfp, d_bfq, d_atp, d_atq, d_pmol, d_off_full, d_n_full,
-        d_off_skip, d_n_skip, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
+    DBuf<int> d_rec0, d_rec1, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
         d_atom_off;
     DBuf<long long> d_pair_off, d_mat_off;
     DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol, d_pz, d_pzi, d_pK, d_pPx, d_pPy, d_pPz;
@@ -372,20 +371,27 @@ static void build_prims(tgpu_ctx *c)
     const size_t np = c->tonto.size();
     c->off_full.assign(np, 0); c->n_full.assign(np, 0);
     c->off_skip.assign(np, 0); c->n_skip.assign(np, 0);
-    for (size_t s = 0; s < np; ++s) {
-        const int m = c->pmol[s], s0 = c->shell_off[m];
-        const int ab = c->tonto[s];
-        int a = (int)((std::sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
-        while (a * (a + 1) / 2 > ab) --a;
-        while ((a + 1) * (a + 2) / 2 <= ab) ++a;
-        const int b = ab - a * (a + 1) / 2 + s0;
-        a += s0;
-        const double *PA = &c->xyz[3 * c->sh_atom[a]], *PB = &c->xyz[3 * c->sh_atom[b]];
-        const bool same = c->sh_atom[a] == c->sh_atom[b];
-        const double ABv[3] = {PA[0] - PB[0], PA[1] - PB[1], PA[2] - PB[2]};
-        const double r2 = ABv[0] * ABv[0] + ABv[1] * ABv[1] + ABv[2] * ABv[2];
-        for (int pass = 0; pass < (same ? 1 : 2); ++pass) {
-            const bool skip = pass == 1;
+    // all full runs in pair order, then all skipped runs: neighbouring lanes of a
+    // wavefront (neighbouring ket pairs) then read neighbouring primitive records
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool skip = pass == 1;
+        for (size_t s = 0; s < np; ++s) {
+            const int m = c->pmol[s], s0 = c->shell_off[m];
+            const int ab = c->tonto[s];
+            int a = (int)((std::sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
+            while (a * (a + 1) / 2 > ab) --a;
+            while ((a + 1) * (a + 2) / 2 <= ab) ++a;
+            const int b = ab - a * (a + 1) / 2 + s0;
+            a += s0;
+            const double *PA = &c->xyz[3 * c->sh_atom[a]], *PB = &c->xyz[3 * c->sh_atom[b]];
+            const bool same = c->sh_atom[a] == c->sh_atom[b];
+            if (skip && same) {  // never used: a quartet with a same-atom pair keeps full lists
+                c->off_skip[s] = c->off_full[s];
+                c->n_skip[s] = c->n_full[s];
+                continue;
+            }
+            const double ABv[3] = {PA[0] - PB[0], PA[1] - PB[1], PA[2] - PB[2]};
+            const double r2 = ABv[0] * ABv[0] + ABv[1] * ABv[1] + ABv[2] * ABv[2];
             const int off = (int)c->pz.size();
             for (int bg = 0; bg < c->sh_np[b]; ++bg) {
                 const double be = c->exps[c->sh_off[b] + bg], bcc = c->coefs[c->sh_off[b] + bg];
@@ -416,19 +422,33 @@ static void build_prims(tgpu_ctx *c)
             if (!skip) { c->off_full[s] = off; c->n_full[s] = n; }
             else { c->off_skip[s] = off; c->n_skip[s] = n; }
         }
-        if (same) { c->off_skip[s] = c->off_full[s]; c->n_skip[s] = c->n_full[s]; }
+    }
+}
+
+// packed per-pair records (see DevPairs)
+static void pack_records(const tgpu_ctx *c, std::vector<int> &r0, std::vector<int> &r1)
+{
+    const size_t np = c->tonto.size();
+    r0.resize(4 * np);
+    r1.resize(4 * np);
+    for (size_t s = 0; s < np; ++s) {
+        r0[4 * s + 0] = c->tonto[s];
+        r0[4 * s + 1] = (c->sp[s] << 16) | c->sq[s];
+        r0[4 * s + 2] = (c->atp[s] << 16) | c->atq[s];
+        r0[4 * s + 3] = (c->bfp[s] << 16) | c->bfq[s];
+        r1[4 * s + 0] = c->off_full[s];
+        r1[4 * s + 1] = c->off_skip[s];
+        r1[4 * s + 2] = (c->n_full[s] << 16) | c->n_skip[s];
+        r1[4 * s + 3] = c->pmol[s];
     }
 }
 
 static int upload_pairs(tgpu_ctx *c)
 {
     int rc = 0;
-    if ((rc = dupload(c, c->d_tonto, c->tonto)) || (rc = dupload(c, c->d_sp, c->sp)) ||
-        (rc = dupload(c, c->d_sq, c->sq)) || (rc = dupload(c, c->d_bfp, c->bfp)) ||
-        (rc = dupload(c, c->d_bfq, c->bfq)) || (rc = dupload(c, c->d_atp, c->atp)) ||
-        (rc = dupload(c, c->d_atq, c->atq)) || (rc = dupload(c, c->d_pmol, c->pmol)) ||
-        (rc = dupload(c, c->d_off_full, c->off_full)) || (rc = dupload(c, c->d_n_full, c->n_full)) ||
-        (rc = dupload(c, c->d_off_skip, c->off_skip)) || (rc = dupload(c, c->d_n_skip, c->n_skip)) ||
+    std::vector<int> r0, r1;
+    pack_records(c, r0, r1);
+    if ((rc = dupload(c, c->d_rec0, r0)) || (rc = dupload(c, c->d_rec1, r1)) ||
         (rc = dupload(c, c->d_A, c->A)) || (rc = dupload(c, c->d_AB, c->AB)) ||
         (rc = dupload(c, c->d_pz, c->pz)) || (rc = dupload(c, c->d_pzi, c->pzi)) ||
         (rc = dupload(c, c->d_pK, c->pK)) ||
@@ -445,10 +465,9 @@ static int upload_pairs(tgpu_ctx *c)
 static DevPairs dev_pairs(const tgpu_ctx *c)
 {
     DevPairs d;
-    d.tonto = c->d_tonto.p; d.sh_p = c->d_sp.p; d.sh_q = c->d_sq.p; d.bf_p = c->d_bfp.p;
-    d.bf_q = c->d_bfq.p; d.atom_p = c->d_atp.p; d.atom_q = c->d_atq.p; d.mol = c->d_pmol.p;
-    d.off_full = c->d_off_full.p; d.n_full = c->d_n_full.p; d.off_skip = c->d_off_skip.p;
-    d.n_skip = c->d_n_skip.p; d.A = c->d_A.p; d.AB = c->d_AB.p; d.maxI = c->d_maxI.p;
+    d.rec0 = reinterpret_cast<const int4 *>(c->d_rec0.p);
+    d.rec1 = reinterpret_cast<const int4 *>(c->d_rec1.p);
+    d.A = c->d_A.p; d.AB = c->d_AB.p; d.maxI = c->d_maxI.p;
     return d;
 }
 
@@ -553,6 +572,15 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
     c->atom_off[n_mol] = (int)c->Z.size();
     c->shell_off[n_mol] = (int)c->sh_l.size();
     c->pair_off[n_mol] = c->n_pairs;
+    if ((int)c->Z.size() > 0xffff || c->n_elem > 0 && *std::max_element(c->nbf.begin(), c->nbf.end()) > 0xffff) {
+        c->err = "more than 65535 atoms or basis functions per molecule";
+        return TGPU_E_UNSUPPORTED;
+    }
+    for (int m = 0; m < n_mol; ++m)
+        if (c->shell_off[m + 1] - c->shell_off[m] > 0xffff) {
+            c->err = "more than 65535 shells in a molecule";
+            return TGPU_E_UNSUPPORTED;
+        }
     if (c->n_pairs > 0x7fffffffLL) {
         c->err = "too many shell pairs for 32-bit pair indexing";
         return TGPU_E_UNSUPPORTED;
@@ -635,9 +663,7 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
 {
     if (!c) return TGPU_E_ARG;
     drop_graph(c);
-    dfree(c->d_tonto); dfree(c->d_sp); dfree(c->d_sq); dfree(c->d_bfp); dfree(c->d_bfq);
-    dfree(c->d_atp); dfree(c->d_atq); dfree(c->d_pmol); dfree(c->d_off_full); dfree(c->d_n_full);
-    dfree(c->d_off_skip); dfree(c->d_n_skip); dfree(c->d_sh_l); dfree(c->d_sh_bf);
+    dfree(c->d_rec0); dfree(c->d_rec1); dfree(c->d_sh_l); dfree(c->d_sh_bf);
     dfree(c->d_seg_start); dfree(c->d_seg_count); dfree(c->d_nbf); dfree(c->d_shell_off);
     dfree(c->d_atom_off); dfree(c->d_pair_off); dfree(c->d_mat_off);
     dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_pz); dfree(c->d_pzi);
@@ -747,6 +773,7 @@ extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
             });
         }
         permute_pairs(c, perm);
+        build_prims(c);
         int rc = upload_pairs(c);
         if (rc) return rc;
         c->maxI_tonto.assign(c->n_pairs, 0.0);
