@@ -92,19 +92,24 @@ def gen_class(LP, LQ, LR, LS, R):
     em = Emitter()
     em(f"template <> struct EriBody<{LP}, {LQ}, {LR}, {LS}, {R}> {{", 0)
     em("static constexpr int NP = %d, NQ = %d, NR = %d, NS = %d, NROOT = %d;" % (NP, NQ, NR, NS, nroot))
-    em("static __device__ __forceinline__ void run(const PairView &bra, const PairView &ket, double *__restrict__ out,")
-    em("                                           const double *__restrict__ rys_tab)")
+    em("static __device__ __forceinline__ void run(const DevPrims &pm, const PairView &bra, const PairView &ket,")
+    em("                                           double *__restrict__ out, const double *__restrict__ rys_tab)")
     em("{")
     em(f"{R} G[{NE * NF}];", 2)
     em(f"#pragma unroll", 2)
     em(f"for (int k = 0; k < {NE * NF}; ++k) G[k] = ({R})0;", 2)
-    em("for (int ip = 0; ip < bra.n; ++ip) {", 2)
-    em("const double za = bra.zeta[ip], iza = bra.zinv[ip], Ka = bra.K[ip];", 3)
-    em("const double Px = bra.Px[ip], Py = bra.Py[ip], Pz = bra.Pz[ip];", 3)
-    em(f"const {R} PAx = ({R})(Px - bra.Ax), PAy = ({R})(Py - bra.Ay), PAz = ({R})(Pz - bra.Az);", 3)
-    em("for (int iq = 0; iq < ket.n; ++iq) {", 3)
-    em("const double zc = ket.zeta[iq], izc = ket.zinv[iq], Kc = ket.K[iq];", 4)
-    em("const double Qx = ket.Px[iq], Qy = ket.Py[iq], Qz = ket.Pz[iq];", 4)
+    # outer loop over the ket's primitives (per-lane data, read once each), inner loop over
+    # the bra's primitives (the same for the whole wavefront, so those loads hit one line)
+    em("for (int iq = 0; iq < ket.n; ++iq) {", 2)
+    em("const int iqq = ket.off + iq;", 3)
+    em("const double zc = pm.zeta[iqq], izc = pm.zinv[iqq], Kc = pm.K[iqq];", 3)
+    em("const double Qx = pm.Px[iqq], Qy = pm.Py[iqq], Qz = pm.Pz[iqq];", 3)
+    em(f"const {R} QCx = ({R})(Qx - ket.Ax), QCy = ({R})(Qy - ket.Ay), QCz = ({R})(Qz - ket.Az);", 3)
+    em("for (int ip = 0; ip < bra.n; ++ip) {", 3)
+    em("const int ipp = bra.off + ip;", 4)
+    em("const double za = pm.zeta[ipp], iza = pm.zinv[ipp], Ka = pm.K[ipp];", 4)
+    em("const double Px = pm.Px[ipp], Py = pm.Py[ipp], Pz = pm.Pz[ipp];", 4)
+    em(f"const {R} PAx = ({R})(Px - bra.Ax), PAy = ({R})(Py - bra.Ay), PAz = ({R})(Pz - bra.Az);", 4)
     em("const double izs = 1.0 / (za + zc), rho = za * zc * izs;", 4)
     em("const double QPx = Qx - Px, QPy = Qy - Py, QPz = Qz - Pz;", 4)
     em("const double X = rho * (QPx * QPx + QPy * QPy + QPz * QPz);", 4)
@@ -113,7 +118,6 @@ def gen_class(LP, LQ, LR, LS, R):
     em("const double pref = TWOPI5ON2 * Ka * Kc * sqrt(rho);", 4)
     em("const double rz0 = zc * izs, re0 = za * izs;   // rho/zeta, rho/eta", 4)
     em(f"const {R} hz = ({R})(0.5 * iza), he = ({R})(0.5 * izc), hs = ({R})(0.5 * izs);", 4)
-    em(f"const {R} QCx = ({R})(Qx - ket.Ax), QCy = ({R})(Qy - ket.Ay), QCz = ({R})(Qz - ket.Az);", 4)
     em(f"const {R} qpx = ({R})QPx, qpy = ({R})QPy, qpz = ({R})QPz;", 4)
     em(f"#pragma unroll", 4)
     em(f"for (int r = 0; r < {nroot}; ++r) {{", 4)

@@ -14,9 +14,15 @@
 // ----------------------------------------------------------------------------------
 // Device helpers
 // ----------------------------------------------------------------------------------
-struct PairView {
+struct DevPrims {
     const double *zeta, *zinv, *K, *Px, *Py, *Pz;
-    int n;
+};
+
+// One shell pair as the integral code sees it: a run of n primitive pairs starting at
+// off in the primitive arrays (DevPrims), and the pair's A and A - B.  A 32-bit offset
+// rather than six pointers keeps the per-lane state small.
+struct PairView {
+    int off, n;
     double Ax, Ay, Az, ABx, ABy, ABz;
 };
 
@@ -143,9 +149,6 @@ struct DevPairs {
 };
 __device__ __forceinline__ int hi16(int v) { return (int)((unsigned)v >> 16); }
 __device__ __forceinline__ int lo16(int v) { return v & 0xffff; }
-struct DevPrims {
-    const double *zeta, *zinv, *K, *Px, *Py, *Pz;
-};
 
 __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims &pm, int s,
                                               bool skip)
@@ -154,12 +157,7 @@ __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims
     const int4 r1 = pr.rec1[s];
     const int off = skip ? r1.y : r1.x;
     v.n = skip ? lo16(r1.z) : hi16(r1.z);
-    v.zeta = pm.zeta + off;
-    v.zinv = pm.zinv + off;
-    v.K = pm.K + off;
-    v.Px = pm.Px + off;
-    v.Py = pm.Py + off;
-    v.Pz = pm.Pz + off;
+    v.off = off;
     v.Ax = pr.A[3 * s];
     v.Ay = pr.A[3 * s + 1];
     v.Az = pr.A[3 * s + 2];
@@ -284,11 +282,19 @@ __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int 
     return true;
 }
 
+// Occupancy floor handed to the register allocator: (ss|ss) fits 8 wavefronts per SIMD
+// (<= 64 VGPRs) without spilling; every other class spills when pushed above what it
+// reaches unconstrained, so it is left free.
+__host__ __device__ constexpr int jk_min_waves(int lp, int lq, int lr, int ls)
+{
+    return lp + lq + lr + ls == 0 ? 8 : 1;
+}
+
 // ----------------------------------------------------------------------------------
 // The J/K kernel for one light class pair (LP LQ | LR LS): one lane per quartet
 // ----------------------------------------------------------------------------------
 template <int LP, int LQ, int LR, int LS, bool MIXED>
-__global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves(LP, LQ, LR, LS), 8))) void jk_kernel(JKArgs a)
 {
     constexpr int NP = ncart(LP), NQ = ncart(LQ), NR = ncart(LR), NS = ncart(LS);
     constexpr int NPQ = NP * NQ, NRS = NR * NS, NI = NPQ * NRS;
@@ -300,8 +306,10 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
         rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= a.n_bra) return;  // wave-uniform
+    // the bra pair is uniform over the wavefront: readfirstlane lets the compiler keep
+    // everything derived from it in scalar registers
+    const int i = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (i >= a.n_bra) return;
     BraInfo b;
     load_bra(a, i, b);
     const int ld = b.ld, fp = b.fp, fq = b.fq;
@@ -333,9 +341,9 @@ __global__ __launch_bounds__(256) void jk_kernel(JKArgs a)
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         double I[NI];
         if (!MIXED || j < jsplit) {
-            EriBody<LP, LQ, LR, LS, double>::run(bv, kv, I, rys_lds);
+            EriBody<LP, LQ, LR, LS, double>::run(a.pm, bv, kv, I, rys_lds);
         } else {
-            EriBody<LP, LQ, LR, LS, float>::run(bv, kv, I, rys_lds);
+            EriBody<LP, LQ, LR, LS, float>::run(a.pm, bv, kv, I, rys_lds);
             ++n32;
         }
         ++nq;
@@ -465,7 +473,7 @@ __global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, i
     const int s = start + i;
     const PairView v = make_view(pr, pm, s, false);
     double I[NP * NQ * NP * NQ];
-    EriBody<LP, LQ, LP, LQ, double>::run(v, v, I, rys_table_data + rys_table_offset[EriBody<LP, LQ, LP, LQ, double>::NROOT]);
+    EriBody<LP, LQ, LP, LQ, double>::run(pm, v, v, I, rys_table_data + rys_table_offset[EriBody<LP, LQ, LP, LQ, double>::NROOT]);
     double m = 0.0;
 #pragma unroll
     for (int p = 0; p < NP; ++p)
@@ -494,7 +502,8 @@ struct WaveLDS {
 // Computes (pq|rs) of one contracted quartet into LDS; returns the buffer holding
 // [pq][rs].  Must be called by all 64 lanes of the work-group.
 template <int LP, int LQ, int LR, int LS>
-__device__ const double *wave_eri(const PairView &bra, const PairView &ket, WaveLDS<LP, LQ, LR, LS> &L)
+__device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const PairView &ket,
+                                  WaveLDS<LP, LQ, LR, LS> &L)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
     constexpr int N = T::NROOT, W1 = T::LPQ + 1, W2 = T::LRS + 1, TW = W1 * W2;
@@ -514,12 +523,14 @@ __device__ const double *wave_eri(const PairView &bra, const PairView &ket, Wave
     }
     const int ax = lane / N, rr = lane - (lane / N) * N;
     for (int ip = 0; ip < bra.n; ++ip) {
-        const double za = bra.zeta[ip], Ka = bra.K[ip];
-        const double Px = bra.Px[ip], Py = bra.Py[ip], Pz = bra.Pz[ip];
+        const int ipp = bra.off + ip;
+        const double za = pm.zeta[ipp], Ka = pm.K[ipp];
+        const double Px = pm.Px[ipp], Py = pm.Py[ipp], Pz = pm.Pz[ipp];
         for (int iq = 0; iq < ket.n; ++iq) {
             if (lane < 3 * N) {
-                const double zc = ket.zeta[iq], Kc = ket.K[iq];
-                const double Qx = ket.Px[iq], Qy = ket.Py[iq], Qz = ket.Pz[iq];
+                const int iqq = ket.off + iq;
+                const double zc = pm.zeta[iqq], Kc = pm.K[iqq];
+                const double Qx = pm.Px[iqq], Qy = pm.Py[iqq], Qz = pm.Pz[iqq];
                 const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
                 const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
                 const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
@@ -633,7 +644,7 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
         for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fr + u / NQ) * ld + fq + u % NQ];
         for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fs + u / NP) * ld + fp + u % NP];
         for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fr + u / NP) * ld + fp + u % NP];
-        const double *I = wave_eri<LP, LQ, LR, LS>(bv, kv, L);
+        const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L);
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
         if (!t.skipJ) {
@@ -706,7 +717,7 @@ __global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims 
     if (i >= n) return;
     const int s = start + i;
     const PairView v = make_view(pr, pm, s, false);
-    const double *I = wave_eri<LP, LQ, LP, LQ>(v, v, L);
+    const double *I = wave_eri<LP, LQ, LP, LQ>(pm, v, v, L);
     double m = 0.0;
     for (int t = threadIdx.x; t < NPQ; t += 64) m = fmax(m, fabs(I[t * NPQ + t]));
 #pragma unroll
