@@ -58,6 +58,22 @@ def test_water_all_classes(torch_cuda, oracle_mod, basis):
     _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 1))
 
 
+def test_f_shells_water_cc_pvtz(torch_cuda, oracle_mod):
+    """cc-pVTZ water: f on O, d on H; (ff|ff) and every other f class pair, including
+    the classes whose transfer buffers live in the global workspace."""
+    from tonto_amd.geometry import tonto_water
+    mol = tonto_water("cc-pVTZ")
+    _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 11))
+
+
+def test_f_shells_water_dimer_cc_pvtz_skip_path(torch_cuda, oracle_mod):
+    """(H2O)_2 / cc-pVTZ: f-shell quartets over four distinct atoms (skipped primitive-pair
+    lists live) at the tight Schwarz cutoff of the (H2O)_64 workload."""
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(2, "cc-pVTZ")
+    _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 12, 0.5), cut=1e-11)
+
+
 def test_four_distinct_atoms_skip_path(torch_cuda, oracle_mod):
     from tonto_amd.geometry import cn_hcn_dimer
     mol = cn_hcn_dimer()

@@ -102,3 +102,32 @@ def test_rys_quad_and_table_integrals_agree(oracle_mod):
     a = o.eri_quartet(*q)
     b = o.eri_quartet(*q, rys_quad=True)
     assert np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(b))
+
+
+def _f_system():
+    # cc-pVTZ: f shells on the two O centres, d shells on the H centres too
+    return Molecule.from_angstrom(["O", "H", "O", "H"],
+                                  [[0.0, 0.0, 0.0], [0.7, 0.5, -0.2], [-0.4, 1.1, 0.6],
+                                   [0.9, -0.8, 0.7]], "cc-pVTZ")
+
+
+def _pick_last(m, l, atom):
+    # the last shell of angular momentum l on the atom (an uncontracted one)
+    return [i for i, s in enumerate(m.shells) if s.l == l and s.atom == atom][-1]
+
+
+@pytest.mark.parametrize("ls", [(3, 0, 0, 0), (0, 3, 0, 0), (3, 0, 3, 0), (3, 3, 0, 0),
+                                (3, 1, 2, 0), (3, 2, 1, 1), (1, 3, 0, 2), (3, 3, 1, 0)])
+def test_f_shells_against_mcmurchie_davidson(oracle_mod, ls):
+    m = _f_system()
+    o = oracle_mod.OracleSystem(m)
+    # f shells sit on the O centres (0, 2); other shells on any of the four
+    atoms = [0, 1, 2, 3]
+    idx = []
+    for l, at in zip(ls, atoms):
+        at = [0, 2][at % 2] if l == 3 else at
+        idx.append(_pick_last(m, l, at))
+    got = o.eri_quartet(*idx)
+    ref = md_eri.shell_quartet(m.shells, *idx)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(got - ref)) < 1e-12 * scale, (ls, idx)

@@ -13,6 +13,7 @@
 
 #define TGPU_LAUNCH_NAME_X(a, b, c, d) TGPU_CAT5(tgpu_launch_jk_, a, b, c, d)
 #define TGPU_SCHWARZ_NAME_X(a, b) TGPU_CAT3(tgpu_launch_schwarz_, a, b)
+#define TGPU_WS_NAME_X(a, b, c, d) TGPU_CAT5(tgpu_wave_ws_, a, b, c, d)
 
 void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int variant, bool mixed,
                                                           hipStream_t s)
@@ -27,19 +28,37 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
 #endif
     (void)variant;
     (void)mixed;  // the wavefront form runs in FP64
-    // one work-group per (bra pair, chunk of a.ket_chunk kets); chunks past a bra pair's
+    using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS>;
+    // one work item per (bra pair, chunk of a.ket_chunk kets); chunks past a bra pair's
     // Schwarz frontier exit at once
-    const int chunks = (a.ket_chunk > 0 ? (a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk : 1);
-    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3(a.n_bra, chunks > 0 ? chunks : 1), 64, 0, s>>>(a);
+    JKArgs b = a;
+    b.n_chunks = (a.ket_chunk > 0 ? (a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk : 1);
+    if (b.n_chunks < 1) b.n_chunks = 1;
+    if (C::GB) {
+        const long long items = (long long)a.n_bra * b.n_chunks;
+        const int grid = (int)(items < a.ws_slots ? items : a.ws_slots);
+        if (grid > 0) jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<grid, 64, 0, s>>>(b);
+    } else {
+        jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3(a.n_bra, b.n_chunks), 64, 0, s>>>(b);
+    }
+}
+
+size_t TGPU_WS_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)()
+{
+    return WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS>::WS;
 }
 
 #if TGPU_LP == TGPU_LR && TGPU_LQ == TGPU_LS
 void TGPU_SCHWARZ_NAME_X(TGPU_LP, TGPU_LQ)(const DevPairs &pr, const DevPrims &pm, int start, int n,
-                                          double *maxI, hipStream_t s)
+                                          double *maxI, double *ws, int ws_slots, hipStream_t s)
 {
 #if TGPU_HEAVY
-    schwarz_wave_kernel<TGPU_LP, TGPU_LQ><<<n, 64, 0, s>>>(pr, pm, start, n, maxI);
+    using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LP, TGPU_LQ>;
+    const int grid = C::GB ? (n < ws_slots ? n : ws_slots) : n;
+    if (grid > 0) schwarz_wave_kernel<TGPU_LP, TGPU_LQ><<<grid, 64, 0, s>>>(pr, pm, start, n, maxI, ws);
 #else
+    (void)ws;
+    (void)ws_slots;
     schwarz_kernel<TGPU_LP, TGPU_LQ><<<(n + 63) / 64, 64, 0, s>>>(pr, pm, start, n, maxI);
 #endif
 }

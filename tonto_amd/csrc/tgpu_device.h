@@ -65,6 +65,34 @@ __device__ __forceinline__ void rys_roots(double X, double *u, double *w)
     }
 }
 
+// Root r alone (u_r, w_r) of the n-root rule: two Clenshaw chains instead of 2N, for
+// code where each lane needs only its own root.
+template <int N>
+__device__ __forceinline__ void rys_root_one(double X, int r, double &u, double &w)
+{
+    if (X >= rys_xasym[N]) {
+        const double ix = 1.0 / X;
+        u = rys_herm_s2[(N - 1) * RYS_NMAX + r] * ix;
+        w = rys_herm_w[(N - 1) * RYS_NMAX + r] * sqrt(ix);
+        return;
+    }
+    const int k = (int)(X * (1.0 / RYS_H));
+    const double t = (X - k * RYS_H) * (2.0 / RYS_H) - 1.0, tt = 2.0 * t;
+    const double *c = rys_table_data + rys_table_offset[N] + k * (RYS_NCHEB * 2 * N);
+    double bu1 = 0.0, bu2 = 0.0, bw1 = 0.0, bw2 = 0.0;
+#pragma unroll
+    for (int m = RYS_NCHEB - 1; m >= 1; --m) {
+        const double bu0 = tt * bu1 - bu2 + c[m * 2 * N + r];
+        const double bw0 = tt * bw1 - bw2 + c[m * 2 * N + N + r];
+        bu2 = bu1;
+        bu1 = bu0;
+        bw2 = bw1;
+        bw1 = bw0;
+    }
+    u = t * bu1 - bu2 + c[r];
+    w = t * bw1 - bw2 + c[N + r];
+}
+
 // Same evaluation with the n-root coefficient block passed in (LDS copy or global):
 // tab points at rys_table_data + rys_table_offset[N]; pairs (root, weight) are read as
 // 16-byte vectors.
@@ -192,6 +220,9 @@ struct JKArgs {
     DevPrims pm;
     DevMols ml;
     int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot, ket_chunk, n_ket_max;
+    int n_chunks;  // ket chunks per bra pair (wavefront form; set by the launcher)
+    double *ws;    // wavefront-form workspace: ws_slots slots of the class's WaveCfg::WS doubles
+    int ws_slots;
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -490,36 +521,75 @@ __global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, i
 // the bra-side transfer relation run level by level in LDS with lanes over entries;
 // the digestion gives each J/K target element to one lane (one atomic per element).
 // ----------------------------------------------------------------------------------
+// LDS budget of the wavefront form.  A class whose transfer buffers would take more keeps
+// them in a global workspace slot instead (one slot per resident work-group, persistent
+// grid), so that the 160 KiB of a CU still holds several work-groups.
+#define TGPU_WAVE_LDS_BUDGET (64 * 1024)
+
+template <int LP, int LQ, int LR, int LS>
+struct WaveCfg {
+    using T = WaveTab<LP, LQ, LR, LS>;
+    static constexpr int NG = T::NE * T::NF, MG = (NG + 63) / 64;
+    static constexpr int NGT = 3 * T::NROOT * (T::LPQ + 1) * (T::LRS + 1);
+    static constexpr int NDEN = T::NP * T::NQ + T::NR * T::NS + T::NS * T::NQ + T::NR * T::NQ +
+                                T::NS * T::NP + T::NR * T::NP;
+    // (e0|f0) accumulators in registers (at most 16 per lane); else in the buffer
+    static constexpr bool GREG = MG <= 16;
+    // transfer buffers in the global workspace
+    static constexpr bool GB = (size_t)(NGT + 2 * T::BUF + NDEN) * 8 > TGPU_WAVE_LDS_BUDGET;
+    static constexpr int LBUF = GB ? 1 : T::BUF;
+    // doubles of workspace per slot (0: LDS only)
+    static constexpr size_t WS = GB ? 2 * (size_t)T::BUF : 0;
+};
+
 template <int LP, int LQ, int LR, int LS>
 struct WaveLDS {
     using T = WaveTab<LP, LQ, LR, LS>;
-    double g[3 * T::NROOT * (T::LPQ + 1) * (T::LRS + 1)];
-    double buf[2][T::BUF];
+    using C = WaveCfg<LP, LQ, LR, LS>;
+    double g[C::NGT];
+    double buf[2][C::LBUF];
     double Ppq[T::NP * T::NQ], Prs[T::NR * T::NS];
     double Psq[T::NS * T::NQ], Prq[T::NR * T::NQ], Psp[T::NS * T::NP], Prp[T::NR * T::NP];
 };
 
-// Computes (pq|rs) of one contracted quartet into LDS; returns the buffer holding
-// [pq][rs].  Must be called by all 64 lanes of the work-group.
+// Computes (pq|rs) of one contracted quartet; returns the buffer holding [pq][rs] (LDS,
+// or the work-group's workspace slot gbuf of 2*BUF doubles for the classes that keep
+// their transfer buffers in global memory).  Must be called by all 64 lanes of the
+// work-group.
 template <int LP, int LQ, int LR, int LS>
 __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const PairView &ket,
-                                  WaveLDS<LP, LQ, LR, LS> &L)
+                                  WaveLDS<LP, LQ, LR, LS> &L, double *gbuf)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
+    using C = WaveCfg<LP, LQ, LR, LS>;
     constexpr int N = T::NROOT, W1 = T::LPQ + 1, W2 = T::LRS + 1, TW = W1 * W2;
-    constexpr int NG = T::NE * T::NF, MG = (NG + 63) / 64;
+    constexpr int NG = C::NG, MG = C::GREG ? C::MG : 1;
     constexpr int NRS = T::NR * T::NS;
     const int lane = threadIdx.x;
+    double *src, *dst;
+    if constexpr (C::GB) {
+        src = gbuf;
+        dst = gbuf + T::BUF;
+    } else {
+        src = L.buf[0];
+        dst = L.buf[1];
+        (void)gbuf;
+    }
     double Gr[MG];
     int gi[MG][3];
+    if constexpr (C::GREG) {
 #pragma unroll
-    for (int m = 0; m < MG; ++m) {
-        Gr[m] = 0.0;
-        const int k = lane + 64 * m;
-        const int kk = k < NG ? k : 0;
-        gi[m][0] = T::gidx[3 * kk];
-        gi[m][1] = TW * N + T::gidx[3 * kk + 1];
-        gi[m][2] = 2 * TW * N + T::gidx[3 * kk + 2];
+        for (int m = 0; m < MG; ++m) {
+            Gr[m] = 0.0;
+            const int k = lane + 64 * m;
+            const int kk = k < NG ? k : 0;
+            gi[m][0] = T::gidx[3 * kk];
+            gi[m][1] = TW * N + T::gidx[3 * kk + 1];
+            gi[m][2] = 2 * TW * N + T::gidx[3 * kk + 2];
+        }
+    } else {
+        // each lane owns entries lane, lane+64, ... until the transfer step
+        for (int k = lane; k < NG; k += 64) src[k] = 0.0;
     }
     const int ax = lane / N, rr = lane - (lane / N) * N;
     for (int ip = 0; ip < bra.n; ++ip) {
@@ -534,12 +604,8 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
                 const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
                 const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
                 const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
-                double u[N], w[N];
-                rys_roots<N>(X, u, w);
-                double ur = u[0], wr = w[0];
-#pragma unroll
-                for (int r = 1; r < N; ++r)
-                    if (rr == r) { ur = u[r]; wr = w[r]; }
+                double ur, wr;
+                rys_root_one<N>(X, rr, ur, wr);
                 const double PA = (ax == 0 ? Px - bra.Ax : ax == 1 ? Py - bra.Ay : Pz - bra.Az);
                 const double QC = (ax == 0 ? Qx - ket.Ax : ax == 1 ? Qy - ket.Ay : Qz - ket.Az);
                 const double qp = QP[ax];
@@ -564,22 +630,34 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
                     for (int f = 0; f < W2; ++f) dst[e * W2 + f] = t[e][f];
             }
             __syncthreads();
+            if constexpr (C::GREG) {
 #pragma unroll
-            for (int m = 0; m < MG; ++m) {
-                double acc = 0.0;
+                for (int m = 0; m < MG; ++m) {
+                    double acc = 0.0;
 #pragma unroll
-                for (int r = 0; r < N; ++r)
-                    acc += L.g[gi[m][0] + r * TW] * L.g[gi[m][1] + r * TW] * L.g[gi[m][2] + r * TW];
-                Gr[m] += acc;
+                    for (int r = 0; r < N; ++r)
+                        acc += L.g[gi[m][0] + r * TW] * L.g[gi[m][1] + r * TW] * L.g[gi[m][2] + r * TW];
+                    Gr[m] += acc;
+                }
+            } else {
+                for (int k = lane; k < NG; k += 64) {
+                    const int i0 = T::gidx[3 * k], i1 = TW * N + T::gidx[3 * k + 1],
+                              i2 = 2 * TW * N + T::gidx[3 * k + 2];
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < N; ++r) acc += L.g[i0 + r * TW] * L.g[i1 + r * TW] * L.g[i2 + r * TW];
+                    src[k] += acc;
+                }
             }
             __syncthreads();
         }
     }
-    double *src = L.buf[0], *dst = L.buf[1];
+    if constexpr (C::GREG) {
 #pragma unroll
-    for (int m = 0; m < MG; ++m) {
-        const int k = lane + 64 * m;
-        if (k < NG) src[k] = Gr[m];
+        for (int m = 0; m < MG; ++m) {
+            const int k = lane + 64 * m;
+            if (k < NG) src[k] = Gr[m];
+        }
     }
     __syncthreads();
     // ket-side transfer, rows e (T::NE of them), entries per level
@@ -609,19 +687,18 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
     return src;
 }
 
+// One work item of the wavefront form: bra pair bi against chunk `chunk` of its ket prefix.
 template <int LP, int LQ, int LR, int LS>
-__global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
+__device__ void wave_item(const JKArgs &a, int bi, int chunk, WaveLDS<LP, LQ, LR, LS> &L, double *gbuf)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
     constexpr int NP = T::NP, NQ = T::NQ, NR = T::NR, NS = T::NS;
     constexpr int NPQ = NP * NQ, NRS = NR * NS, MJ = (NPQ + 63) / 64;
-    __shared__ WaveLDS<LP, LQ, LR, LS> L;
     const int lane = threadIdx.x;
-    if ((int)blockIdx.x >= a.n_bra) return;
     BraInfo b;
-    load_bra(a, blockIdx.x, b);
-    // this work-group's chunk of the ket prefix
-    const int j_begin = blockIdx.y * a.ket_chunk;
+    load_bra(a, bi, b);
+    // this work item's chunk of the ket prefix
+    const int j_begin = chunk * a.ket_chunk;
     const int j_end = min(b.jend, j_begin + a.ket_chunk);
     if (j_begin >= j_end) return;
     const int ld = b.ld, fp = b.fp, fq = b.fq;
@@ -644,7 +721,7 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
         for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fr + u / NQ) * ld + fq + u % NQ];
         for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fs + u / NP) * ld + fp + u % NP];
         for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fr + u / NP) * ld + fp + u % NP];
-        const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L);
+        const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L, gbuf);
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
         if (!t.skipJ) {
@@ -706,34 +783,57 @@ __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
     }
 }
 
-template <int LP, int LQ>
-__global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims pm, int start, int n,
-                                                         double *maxI)
+// Wavefront form.  LDS classes: grid (n_bra, n_chunks), one item per work-group.
+// Workspace classes: a persistent 1-D grid of at most ws_slots work-groups; work-group g
+// uses slot g and walks the items g, g + gridDim.x, ...
+template <int LP, int LQ, int LR, int LS>
+__global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
 {
-    using T = WaveTab<LP, LQ, LP, LQ>;
-    constexpr int NPQ = T::NP * T::NQ;
-    __shared__ WaveLDS<LP, LQ, LP, LQ> L;
-    const int i = blockIdx.x;
-    if (i >= n) return;
-    const int s = start + i;
-    const PairView v = make_view(pr, pm, s, false);
-    const double *I = wave_eri<LP, LQ, LP, LQ>(pm, v, v, L);
-    double m = 0.0;
-    for (int t = threadIdx.x; t < NPQ; t += 64) m = fmax(m, fabs(I[t * NPQ + t]));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    if (threadIdx.x == 0) maxI[s] = sqrt(m);
+    using C = WaveCfg<LP, LQ, LR, LS>;
+    __shared__ WaveLDS<LP, LQ, LR, LS> L;
+    if constexpr (C::GB) {
+        const long long items = (long long)a.n_bra * a.n_chunks;
+        double *gbuf = a.ws + (size_t)blockIdx.x * C::WS;
+        for (long long it = blockIdx.x; it < items; it += gridDim.x)
+            wave_item<LP, LQ, LR, LS>(a, (int)(it / a.n_chunks), (int)(it % a.n_chunks), L, gbuf);
+    } else {
+        if ((int)blockIdx.x >= a.n_bra) return;
+        wave_item<LP, LQ, LR, LS>(a, blockIdx.x, blockIdx.y, L, nullptr);
+    }
 }
 
-// Launchers defined in the per-class translation units (jk_class.hip)
+template <int LP, int LQ>
+__global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims pm, int start, int n,
+                                                         double *maxI, double *ws)
+{
+    using T = WaveTab<LP, LQ, LP, LQ>;
+    using C = WaveCfg<LP, LQ, LP, LQ>;
+    constexpr int NPQ = T::NP * T::NQ;
+    __shared__ WaveLDS<LP, LQ, LP, LQ> L;
+    double *gbuf = C::GB ? ws + (size_t)blockIdx.x * C::WS : nullptr;
+    // LDS classes launch one work-group per pair; workspace classes a persistent grid
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int s = start + i;
+        const PairView v = make_view(pr, pm, s, false);
+        const double *I = wave_eri<LP, LQ, LP, LQ>(pm, v, v, L, gbuf);
+        double m = 0.0;
+        for (int t = threadIdx.x; t < NPQ; t += 64) m = fmax(m, fabs(I[t * NPQ + t]));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        if (threadIdx.x == 0) maxI[s] = sqrt(m);
+        __syncthreads();
+    }
+}
 
 // Launchers defined in the per-class translation units (jk_class.hip).  variant 0: one
 // lane per quartet (light classes only); variant 1: one wavefront per quartet.
 #define TGPU_DECLARE_JK(LP, LQ, LR, LS, H) \
-    void tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, int variant, bool mixed, hipStream_t s);
+    void tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, int variant, bool mixed, hipStream_t s); \
+    size_t tgpu_wave_ws_##LP##LQ##LR##LS();
 TGPU_FOR_EACH_CLASS(TGPU_DECLARE_JK)
 #undef TGPU_DECLARE_JK
 #define TGPU_DECLARE_SCHWARZ(LP, LQ, H) \
-    void tgpu_launch_schwarz_##LP##LQ(const DevPairs &pr, const DevPrims &pm, int start, int n, double *maxI, hipStream_t s);
+    void tgpu_launch_schwarz_##LP##LQ(const DevPairs &pr, const DevPrims &pm, int start, int n, double *maxI, \
+                                      double *ws, int ws_slots, hipStream_t s);
 TGPU_FOR_EACH_PAIRCLASS(TGPU_DECLARE_SCHWARZ)
 #undef TGPU_DECLARE_SCHWARZ

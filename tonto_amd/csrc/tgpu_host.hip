@@ -304,6 +304,9 @@ struct tgpu_ctx {
     DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol, d_pz, d_pzi, d_pK, d_pPx, d_pPy, d_pPz;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
     DBuf<unsigned long long> d_counters;
+    // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
+    DBuf<double> d_ws;
+    int ws_slots = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
     bool timed = false;
     bool profiling = false;
@@ -670,6 +673,7 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_pK); dfree(c->d_pPx); dfree(c->d_pPy); dfree(c->d_pPz); dfree(c->d_X); dfree(c->d_Y);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
     dfree(c->d_counters);
+    dfree(c->d_ws);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
@@ -743,16 +747,50 @@ extern "C" int tgpu_set_fp32_threshold(tgpu_ctx *c, double thr)
     return TGPU_OK;
 }
 
+typedef size_t (*ws_query)();
+static const ws_query kWaveWs[NQCLS] = {
+#define TGPU_WS_FN(LP, LQ, LR, LS, H) tgpu_wave_ws_##LP##LQ##LR##LS,
+    TGPU_FOR_EACH_CLASS(TGPU_WS_FN)
+#undef TGPU_WS_FN
+};
+static const int kClassPair[NQCLS][2] = {
+#define TGPU_CLASSPAIR(LP, LQ, LR, LS, H) {pair_class(LP, LQ), pair_class(LR, LS)},
+    TGPU_FOR_EACH_CLASS(TGPU_CLASSPAIR)
+#undef TGPU_CLASSPAIR
+};
+
+// Work-groups (and workspace slots) of a workspace-class launch: 4 per CU of an MI355X
+static const int kWsSlots = 1024;
+
+// Allocates the wavefront-form workspace for the class pairs this basis has (none for
+// s/p/d bases): kWsSlots slots of the largest per-slot need.
+static int ensure_workspace(tgpu_ctx *c)
+{
+    size_t need = 0;
+    for (int k = 0; k < NQCLS; ++k) {
+        const int CX = kClassPair[k][0], CY = kClassPair[k][1];
+        const int nb = c->bin_start[CX * NBK + NBK] - c->bin_start[CX * NBK];
+        const int nk = c->bin_start[CY * NBK + NBK] - c->bin_start[CY * NBK];
+        if (nb && nk) need = std::max(need, kWaveWs[k]());
+    }
+    if (!need) return 0;
+    if (int rc = dalloc(c, c->d_ws, need * kWsSlots)) return rc;
+    c->ws_slots = kWsSlots;
+    return 0;
+}
+
 extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
 {
     if (!c) return TGPU_E_ARG;
     if (!c->have_schwarz) {
+        if (int rc = ensure_workspace(c)) return rc;
         const DevPairs dpr = dev_pairs(c);
         const DevPrims dpm = dev_prims(c);
 #define TGPU_SCHWARZ_CASE(LP, LQ, H)                                                       \
     {                                                                                    \
         const int b0 = pair_class(LP, LQ) * NBK, n = c->bin_start[b0 + NBK] - c->bin_start[b0]; \
-        if (n) tgpu_launch_schwarz_##LP##LQ(dpr, dpm, c->bin_start[b0], n, c->d_maxI.p, c->stream); \
+        if (n) tgpu_launch_schwarz_##LP##LQ(dpr, dpm, c->bin_start[b0], n, c->d_maxI.p, c->d_ws.p, \
+                                            c->ws_slots, c->stream);                          \
     }
         TGPU_FOR_EACH_PAIRCLASS(TGPU_SCHWARZ_CASE)
 #undef TGPU_SCHWARZ_CASE
@@ -796,12 +834,6 @@ static const jk_launcher kLaunch[NQCLS] = {
     TGPU_FOR_EACH_CLASS(TGPU_LAUNCHER)
 #undef TGPU_LAUNCHER
 };
-static const int kClassPair[NQCLS][2] = {
-#define TGPU_CLASSPAIR(LP, LQ, LR, LS, H) {pair_class(LP, LQ), pair_class(LR, LS)},
-    TGPU_FOR_EACH_CLASS(TGPU_CLASSPAIR)
-#undef TGPU_CLASSPAIR
-};
-
 static int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
 
 // Enqueue one complete build on stream s (no host synchronisation).
@@ -834,6 +866,9 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.X = c->d_X.p;
     a.Y = c->d_Y.p;
     a.counters = c->d_counters.p;
+    a.ws = c->d_ws.p;
+    a.ws_slots = c->ws_slots;
+    a.n_chunks = 1;
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];
         const bool prof = timing && c->profiling;

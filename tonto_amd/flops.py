@@ -15,7 +15,7 @@ the per-class counters the library returns (tgpu_get_class_counters).
 """
 from .basis import n_comp
 
-LMAX = 2
+LMAX = 3
 
 
 def comps_range(l0, l1):
