@@ -1,6 +1,8 @@
 // One angular-momentum class pair of the J/K build per translation unit, so that the
 // classes compile in parallel.  Built with -DTGPU_LP=.. -DTGPU_LQ=.. -DTGPU_LR=..
 // -DTGPU_LS=.. -DTGPU_HEAVY=0|1.
+#include <algorithm>
+
 #include "tgpu_device.h"
 
 #define TGPU_STR2(x) #x
@@ -29,18 +31,25 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
     (void)variant;
     (void)mixed;  // the wavefront form runs in FP64
     using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS>;
-    // one work item per (bra pair, chunk of a.ket_chunk kets); chunks past a bra pair's
-    // Schwarz frontier exit at once
-    JKArgs b = a;
-    b.n_chunks = (a.ket_chunk > 0 ? (a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk : 1);
-    if (b.n_chunks < 1) b.n_chunks = 1;
+    // work-group (x, y): bra pairs x, x + gx, ...; of each, ket chunks y, y + gy, ... of
+    // a.ket_chunk kets.  gy splits long prefixes when there are few bra pairs, so that the
+    // grid holds about kWaveGrid work-groups; workspace classes stay within ws_slots.
+    constexpr long long kWaveGrid = 65536;
+    if (a.n_bra <= 0) return;
+    const int kc = a.ket_chunk > 0 ? a.ket_chunk : 16;
+    const long long nch = a.n_ket_max > 0 ? (a.n_ket_max + kc - 1) / kc : 1;
+    long long gx = a.n_bra, gy;
     if (C::GB) {
-        const long long items = (long long)a.n_bra * b.n_chunks;
-        const int grid = (int)(items < a.ws_slots ? items : a.ws_slots);
-        if (grid > 0) jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<grid, 64, 0, s>>>(b);
+        if (a.ws_slots <= 0 || !a.ws) return;  // unreachable: tgpu_schwarz allocates it first
+        gx = std::min<long long>(gx, a.ws_slots);
+        gy = std::min<long long>(nch, std::max<long long>(1, a.ws_slots / gx));
     } else {
-        jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3(a.n_bra, b.n_chunks), 64, 0, s>>>(b);
+        gy = std::min<long long>(nch, std::max<long long>(1, (kWaveGrid + gx - 1) / gx));
     }
+    gy = std::min<long long>(gy, 65535);
+    JKArgs b = a;
+    b.ket_chunk = kc;
+    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3((unsigned)gx, (unsigned)gy), 64, 0, s>>>(b);
 }
 
 size_t TGPU_WS_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)()

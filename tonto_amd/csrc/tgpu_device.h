@@ -220,7 +220,6 @@ struct JKArgs {
     DevPrims pm;
     DevMols ml;
     int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot, ket_chunk, n_ket_max;
-    int n_chunks;  // ket chunks per bra pair (wavefront form; set by the launcher)
     double *ws;    // wavefront-form workspace: ws_slots slots of the class's WaveCfg::WS doubles
     int ws_slots;
     const double *P, *Pmax, *scal;
@@ -687,9 +686,10 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
     return src;
 }
 
-// One work item of the wavefront form: bra pair bi against chunk `chunk` of its ket prefix.
+// The wavefront form's work for bra pair bi: the chunks blockIdx.y, blockIdx.y + gridDim.y,
+// ... of a.ket_chunk kets of its Schwarz prefix (only chunks that hold kets are visited).
 template <int LP, int LQ, int LR, int LS>
-__device__ void wave_item(const JKArgs &a, int bi, int chunk, WaveLDS<LP, LQ, LR, LS> &L, double *gbuf)
+__device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, double *gbuf)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
     constexpr int NP = T::NP, NQ = T::NQ, NR = T::NR, NS = T::NS;
@@ -697,10 +697,8 @@ __device__ void wave_item(const JKArgs &a, int bi, int chunk, WaveLDS<LP, LQ, LR
     const int lane = threadIdx.x;
     BraInfo b;
     load_bra(a, bi, b);
-    // this work item's chunk of the ket prefix
-    const int j_begin = chunk * a.ket_chunk;
-    const int j_end = min(b.jend, j_begin + a.ket_chunk);
-    if (j_begin >= j_end) return;
+    const int step = gridDim.y * a.ket_chunk;
+    if ((int)blockIdx.y * a.ket_chunk >= b.jend) return;  // uniform over the work-group
     const int ld = b.ld, fp = b.fp, fq = b.fq;
     const double *__restrict__ P = b.P;
     for (int t = lane; t < NPQ; t += 64) L.Ppq[t] = P[(size_t)(fp + t / NQ) * ld + fq + t % NQ];
@@ -708,7 +706,8 @@ __device__ void wave_item(const JKArgs &a, int bi, int chunk, WaveLDS<LP, LQ, LR
 #pragma unroll
     for (int m = 0; m < MJ; ++m) Jpq[m] = 0.0;
     unsigned long long nq = 0, npq = 0;
-    for (int j = j_begin; j < j_end; ++j) {  // uniform over the work-group
+    for (int jc = blockIdx.y * a.ket_chunk; jc < b.jend; jc += step)
+    for (int j = jc; j < min(b.jend, jc + a.ket_chunk); ++j) {  // uniform over the work-group
         KetTest t;
         if (!test_ket(a, b, j, t)) continue;
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
@@ -781,25 +780,20 @@ __device__ void wave_item(const JKArgs &a, int bi, int chunk, WaveLDS<LP, LQ, LR
         atomicAdd(&a.counters[3 * a.counter_slot + 0], nq);
         atomicAdd(&a.counters[3 * a.counter_slot + 1], npq);
     }
+    __syncthreads();  // L is reused by the work-group's next bra pair
 }
 
-// Wavefront form.  LDS classes: grid (n_bra, n_chunks), one item per work-group.
-// Workspace classes: a persistent 1-D grid of at most ws_slots work-groups; work-group g
-// uses slot g and walks the items g, g + gridDim.x, ...
+// Wavefront form: grid (gx, gy); work-group (x, y) takes bra pairs x, x + gx, ... and, of
+// each, the ket chunks y, y + gy, ...  Workspace classes keep gx * gy <= ws_slots and use
+// slot y * gx + x.
 template <int LP, int LQ, int LR, int LS>
 __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
 {
     using C = WaveCfg<LP, LQ, LR, LS>;
     __shared__ WaveLDS<LP, LQ, LR, LS> L;
-    if constexpr (C::GB) {
-        const long long items = (long long)a.n_bra * a.n_chunks;
-        double *gbuf = a.ws + (size_t)blockIdx.x * C::WS;
-        for (long long it = blockIdx.x; it < items; it += gridDim.x)
-            wave_item<LP, LQ, LR, LS>(a, (int)(it / a.n_chunks), (int)(it % a.n_chunks), L, gbuf);
-    } else {
-        if ((int)blockIdx.x >= a.n_bra) return;
-        wave_item<LP, LQ, LR, LS>(a, blockIdx.x, blockIdx.y, L, nullptr);
-    }
+    double *gbuf = nullptr;
+    if constexpr (C::GB) gbuf = a.ws + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * C::WS;
+    for (int bi = blockIdx.x; bi < a.n_bra; bi += gridDim.x) wave_bra<LP, LQ, LR, LS>(a, bi, L, gbuf);
 }
 
 template <int LP, int LQ>

@@ -868,7 +868,6 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.counters = c->d_counters.p;
     a.ws = c->d_ws.p;
     a.ws_slots = c->ws_slots;
-    a.n_chunks = 1;
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];
         const bool prof = timing && c->profiling;
