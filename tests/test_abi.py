@@ -1,5 +1,5 @@
 """The C-ABI library loads without a GPU and exports every function that
-include/tonto_gpu.h declares (no compute calls here)."""
+include/tonto_gpu.h and include/tonto_mlp.h declare (no compute calls here)."""
 import ctypes
 import os
 import re
@@ -9,10 +9,10 @@ from tonto_amd import gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    txt = open(os.path.join(ROOT, "include", "tonto_gpu.h")).read()
+def declared_functions(header="tonto_gpu.h", prefix="tgpu_"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(tgpu_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", txt)))
 
 
 def test_header_declares_the_boundary():
@@ -24,7 +24,9 @@ def test_header_declares_the_boundary():
 
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(gpu.LIB_PATH)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    names = declared_functions() + declared_functions("tonto_mlp.h", "tmlp_")
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 

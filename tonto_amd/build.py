@@ -71,6 +71,8 @@ def build(jobs=None, incremental=False, verbose=True):
                        f"TGPU_HEAVY={heavy}"], _deps([gen, os.path.join(CSRC, "jk_class.hip")])))
     tasks.append((os.path.join(CSRC, "tgpu_host.hip"), os.path.join(BUILD, "host.o"), [],
                   _deps([os.path.join(CSRC, "tgpu_host.hip")])))
+    tasks.append((os.path.join(CSRC, "mlp.hip"), os.path.join(BUILD, "mlp.o"), [],
+                  [os.path.join(CSRC, "mlp.hip"), os.path.join(HERE, "..", "include", "tonto_mlp.h")]))
     # largest classes first so the tail is short
     tasks.sort(key=lambda t: -os.path.getsize(next((d for d in t[3] if "eri_" in d), t[0])))
     t0 = time.time()

@@ -11,6 +11,8 @@ from .basis import Molecule
 from .gpu import JKEngine
 from .scf import atomic_density, promolecule_guess, run_rhf
 
+_ATOM_DENSITY = {}
+
 
 class GPUSCF:
     def __init__(self, mol, schwarz=1e-9, j_density=1e-9, k_density=1e-9, pair=1e-6,
@@ -68,10 +70,36 @@ class GPUSCF:
         return atomic_density(torch.tensor(T + V, device=self.device), torch.tensor(S, device=self.device),
                               ELEMENT_Z[symbol], jk)
 
-    def guess(self, kind="promolecule"):
+    def atom_density(self, symbol):
+        """Promolecule block of one element (numpy; cached per basis and element)."""
+        key = (self.mol.basis_name, symbol)
+        if key not in _ATOM_DENSITY:
+            _ATOM_DENSITY[key] = self.atom_scf(symbol).cpu().numpy()
+        return _ATOM_DENSITY[key]
+
+    def guess(self, kind="promolecule", model=None):
+        """Initial density: "core" (None: run_rhf diagonalises h), "promolecule", or "mlp"
+        (the learned guess; falls back to the promolecule guess for molecules or bases it
+        does not cover)."""
         if kind == "core":
             return None
-        return promolecule_guess(self.mol, self.atom_scf)
+        if kind == "mlp":
+            P = self.mlp_guess(model)
+            if P is not None:
+                return P
+        return promolecule_guess(self.mol, lambda sym: torch.tensor(self.atom_density(sym),
+                                                                    device=self.device))
+
+    def mlp_guess(self, model=None):
+        from . import guess_mlp as G
+        model = model or G.default_model()
+        if model is None or str(model.meta.get("basis", "")) != self.mol.basis_name or \
+                not set(self.mol.symbols) <= {"O", "H"}:
+            return None
+        atom = {sym: self.atom_density(sym) for sym in set(self.mol.symbols)}
+        P = G.assemble(self.mol, atom, model.predictor(G.gpu_forward(model, self.device)))
+        P = G.normalise_electrons(P, self.S.cpu().numpy(), self.mol.n_electrons)
+        return torch.tensor(P, device=self.device)
 
     def jk_incremental(self, P, mode):
         """Delta builds are screened with a 1000x tighter Schwarz cutoff than full builds:
@@ -86,10 +114,11 @@ class GPUSCF:
         return self.jk(P, mode)
 
     def run(self, precision="fp64", mixed_until=1e-4, incremental=False, guess="promolecule",
-            log=None, **kw):
+            log=None, guess_model=None, **kw):
+        """RHF to convergence; the wall time includes making the initial guess."""
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
-        P0 = self.guess(guess)
+        P0 = self.guess(guess, guess_model)
         r = run_rhf(self.h, self.S, self.mol.n_electrons, self.mol.nuclear_repulsion(),
                     self.jk_incremental if incremental else self.jk, P0=P0,
                     mixed_until=mixed_until if precision == "mixed" else None,

@@ -1,0 +1,189 @@
+"""Training and evaluation of the learned initial guess (runs on the GPU).
+
+Data: converged RHF/6-31G(d) densities (FP64 GPU SCF from the promolecule guess, energy
+converged to 1e-8 Eh) of seeded water clusters with 4-16 molecules (sizes drawn
+uniformly, seeds seed0, seed0 + 1, ...), turned into per-block training rows by
+tonto_amd.guess_mlp.training_rows.
+
+Training: one network per block type (n_in 273 -> 128 -> 128 -> n_out, SiLU, FP32),
+inputs standardised per feature, targets divided by their RMS; Adam with a cosine
+learning-rate decay, minibatches drawn on the device; every forward, loss, backward and
+update step is a tmlp_* HIP kernel.  5% of each type's rows are held out for the
+validation loss that is printed.
+
+Evaluation: held-out clusters (20 molecules, seeds from 900000): the same FP64 SCF from
+the promolecule guess and from the learned guess, iterations, wall time and the
+difference of the converged energies.
+
+    python -m tonto_amd.guess_train train --clusters 2000 --out tonto_amd/data/guess_mlp_631gd.npz
+    python -m tonto_amd.guess_train eval --clusters 200
+"""
+import argparse
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import guess_mlp as G
+
+BASIS = "6-31G(d)"
+EVAL_SEED0 = 900000
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def generate(n_clusters, nmin, nmax, seed0, log=_log):
+    import torch
+    from .driver import GPUSCF
+    from .geometry import water_cluster
+    rng = np.random.default_rng(seed0)
+    sizes = rng.integers(nmin, nmax + 1, n_clusters)
+    rows = {t: ([], []) for t in G.TYPES}
+    t0 = time.perf_counter()
+    its = []
+    atom = None
+    for i, n in enumerate(sizes):
+        mol = water_cluster(int(n), BASIS, seed=seed0 + i)
+        s = GPUSCF(mol)
+        if atom is None:
+            atom = {sym: s.atom_density(sym) for sym in ("O", "H")}
+        r = s.run(guess="promolecule", convergence=1e-8)
+        if not r.converged:
+            log(f"cluster {i} (n={n}) did not converge; skipped")
+            continue
+        its.append(r.iterations)
+        for t, (x, y) in G.training_rows(mol, r.P.cpu().numpy(), atom).items():
+            rows[t][0].append(x.astype(np.float32))
+            rows[t][1].append(y.astype(np.float32))
+        if (i + 1) % 25 == 0 or i + 1 == n_clusters:
+            log(f"data {i + 1}/{n_clusters} clusters {time.perf_counter() - t0:.1f}s "
+                f"(mean SCF iterations {np.mean(its):.1f})")
+        del s
+        torch.cuda.empty_cache() if (i + 1) % 200 == 0 else None
+    data = {t: (np.concatenate(x), np.concatenate(y)) for t, (x, y) in rows.items() if x}
+    return data, atom, sizes, time.perf_counter() - t0
+
+
+def train_net(t, X, Y, epochs, batch, lr, seed, log=_log):
+    import torch
+    from .mlp import DeviceNet, init_params
+    dev = torch.device("cuda")
+    n_out = Y.shape[1]
+    x_mean = X.mean(0)
+    x_std = X.std(0)
+    x_std[x_std < 1e-6] = 1.0
+    y_scale = float(np.sqrt(np.mean(Y.astype(np.float64) ** 2))) or 1.0
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(X.shape[0])
+    n_val = max(1, X.shape[0] // 20)
+    val, tr = perm[:n_val], perm[n_val:]
+    Xn = ((X - x_mean) / x_std).astype(np.float32)
+    Yn = (Y / y_scale).astype(np.float32)
+    Xt, Yt = torch.tensor(Xn[tr], device=dev), torch.tensor(Yn[tr], device=dev)
+    Xv, Yv = torch.tensor(Xn[val], device=dev), torch.tensor(Yn[val], device=dev)
+    batch = min(batch, Xt.shape[0])
+    steps = Xt.shape[0] // batch
+    net = DeviceNet(G.N_IN, G.NH, n_out, init_params(G.N_IN, G.NH, n_out, seed), dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    hist = []
+    for ep in range(epochs):
+        lr_ep = 0.02 * lr + 0.98 * lr * 0.5 * (1.0 + math.cos(math.pi * ep / epochs))
+        order = torch.randperm(Xt.shape[0], device=dev, generator=gen)
+        tot = torch.zeros((), device=dev)
+        for k in range(steps):
+            idx = order[k * batch:(k + 1) * batch]
+            xb = Xt.index_select(0, idx).contiguous()
+            yb = Yt.index_select(0, idx).contiguous()
+            tot = tot + net.loss_and_grad(xb, yb, 1.0 / (batch * n_out))[0]
+            net.adam(lr_ep)
+        pv = net.forward(Xv)
+        v = float(((pv - Yv) ** 2).mean())
+        hist.append((float(tot) / max(steps, 1), v))
+        if ep % 10 == 0 or ep + 1 == epochs:
+            log(f"{t:2s} epoch {ep:3d} lr {lr_ep:.2e} train {hist[-1][0]:.4e} val {v:.4e}")
+    return dict(params=net.params(), x_mean=x_mean.astype(np.float32), x_std=x_std.astype(np.float32),
+                y_scale=np.float32(y_scale), n_out=n_out), hist
+
+
+def train(args):
+    t0 = time.perf_counter()
+    data, atom, sizes, t_data = generate(args.clusters, args.min, args.max, args.seed0)
+    _log("rows: " + ", ".join(f"{t} {X.shape[0]}x{X.shape[1]}->{Y.shape[1]}" for t, (X, Y) in data.items()))
+    nets, val = {}, {}
+    t1 = time.perf_counter()
+    for k, (t, (X, Y)) in enumerate(data.items()):
+        nets[t], hist = train_net(t, X, Y, args.epochs, args.batch, args.lr, args.seed0 + k)
+        val[t] = hist[-1][1]
+    t_train = time.perf_counter() - t1
+    meta = dict(basis=BASIS, n_clusters=args.clusters, size_min=args.min, size_max=args.max,
+                seed0=args.seed0, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                data_seconds=t_data, train_seconds=t_train,
+                val_loss=np.array([val[t] for t in G.TYPES if t in val]),
+                val_types=np.array([t for t in G.TYPES if t in val]))
+    G.GuessModel(nets, meta).save(args.out)
+    _log(f"saved {args.out}: data {t_data:.0f}s, training {t_train:.0f}s, total {time.perf_counter() - t0:.0f}s")
+
+
+def evaluate(n_clusters, n_waters=20, seed0=EVAL_SEED0, model=None, log=_log):
+    """SCF iterations and wall time from the promolecule and the learned guess on held-out
+    clusters; the converged energies must agree (the guess only moves the iteration count)."""
+    import torch
+    from .driver import GPUSCF
+    from .geometry import water_cluster
+    model = model or G.default_model()
+    if model is None:
+        raise FileNotFoundError(G.DEFAULT_MODEL)
+    out = {"sad_iterations": [], "mlp_iterations": [], "sad_wall": [], "mlp_wall": [], "dE": [],
+           "mlp_guess_seconds": []}
+    for i in range(n_clusters):
+        mol = water_cluster(n_waters, BASIS, seed=seed0 + i)
+        s = GPUSCF(mol)
+        s.atom_density("O"), s.atom_density("H")
+        r0 = s.run(guess="promolecule")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s.guess("mlp", model)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t
+        r1 = s.run(guess="mlp", guess_model=model)
+        out["sad_iterations"].append(r0.iterations)
+        out["mlp_iterations"].append(r1.iterations)
+        out["sad_wall"].append(r0.wall)
+        out["mlp_wall"].append(r1.wall)
+        out["mlp_guess_seconds"].append(tg)
+        out["dE"].append(r1.energy - r0.energy)
+        if (i + 1) % 10 == 0 or i + 1 == n_clusters:
+            log(f"eval {i + 1}/{n_clusters}: iterations promolecule {np.mean(out['sad_iterations']):.2f} "
+                f"learned {np.mean(out['mlp_iterations']):.2f}; max|dE| {np.max(np.abs(out['dE'])):.1e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("train")
+    a.add_argument("--clusters", type=int, default=2000)
+    a.add_argument("--min", type=int, default=4)
+    a.add_argument("--max", type=int, default=16)
+    a.add_argument("--seed0", type=int, default=1000)
+    a.add_argument("--epochs", type=int, default=60)
+    a.add_argument("--batch", type=int, default=2048)
+    a.add_argument("--lr", type=float, default=2e-3)
+    a.add_argument("--out", default=G.DEFAULT_MODEL)
+    e = sub.add_parser("eval")
+    e.add_argument("--clusters", type=int, default=200)
+    e.add_argument("--waters", type=int, default=20)
+    args = ap.parse_args(argv)
+    if args.cmd == "train":
+        train(args)
+    else:
+        r = evaluate(args.clusters, args.waters)
+        print({k: float(np.mean(np.abs(v)) if k == "dE" else np.mean(v)) for k, v in r.items()})
+
+
+if __name__ == "__main__":
+    sys.exit(main())
