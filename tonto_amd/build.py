@@ -58,8 +58,15 @@ def _compile(src, obj, defines, deps, incremental, log):
     return dt
 
 
-def build(jobs=None, incremental=False, verbose=True):
+def build(jobs=None, incremental=False, verbose=True, defines=(), lib=None, build_dir=None):
+    """Compile every translation unit for gfx950 and link the library (default
+    tonto_amd/libtonto_gpu.so).  defines: extra -D macros for all units (variant builds
+    go to their own lib / build_dir)."""
     log = print if verbose else (lambda *a: None)
+    global BUILD
+    lib = lib or LIB
+    if build_dir:
+        BUILD = build_dir
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(16, os.cpu_count() or 8)
     tasks = []
@@ -76,24 +83,28 @@ def build(jobs=None, incremental=False, verbose=True):
     # largest classes first so the tail is short
     tasks.sort(key=lambda t: -os.path.getsize(next((d for d in t[3] if "eri_" in d), t[0])))
     t0 = time.time()
+    tasks = [(src, obj, list(dfs) + list(defines), deps) for (src, obj, dfs, deps) in tasks]
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(lambda t: _compile(*t, incremental, log), tasks))
     objs = [t[1] for t in tasks]
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
-    log(f"built {LIB} in {time.time() - t0:.1f}s")
-    return LIB
+    log(f"built {lib} in {time.time() - t0:.1f}s")
+    return lib
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--incremental", action="store_true")
+    ap.add_argument("--define", action="append", default=[], help="extra NAME=VALUE for every unit")
+    ap.add_argument("--lib", default=None, help="output library (default tonto_amd/libtonto_gpu.so)")
+    ap.add_argument("--build-dir", default=None)
     a = ap.parse_args()
     try:
-        build(a.jobs, a.incremental)
+        build(a.jobs, a.incremental, defines=a.define, lib=a.lib, build_dir=a.build_dir)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

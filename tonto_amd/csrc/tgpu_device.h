@@ -204,6 +204,11 @@ __device__ __forceinline__ long long tonto_pair(int x, int y)
 // (full list) into TGPU_NBUCKET buckets, so that the lanes of a wavefront run primitive
 // loops of similar length; bin = class * TGPU_NBUCKET + bucket.
 #define TGPU_NBUCKET 4
+// lane-per-quartet kernels compact the kets that pass the screening tests (1) or let
+// the lanes whose ket fails idle for that round (0)
+#ifndef TGPU_COMPACT
+#define TGPU_COMPACT 1
+#endif
 #define TGPU_NBIN (TGPU_NPAIRCLASS * TGPU_NBUCKET)
 
 // Per-molecule layout of a batch: molecule m's pairs of bin k occupy
@@ -276,6 +281,7 @@ __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
 struct KetTest {
     int sk, fr, fs;
     bool skipJ, skipK, skip_lists;
+    int halvings;  // permutation factor f = 2^-halvings
     double f;
 };
 
@@ -299,10 +305,8 @@ __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int 
     t.skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
     t.skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
     if (t.skipJ && t.skipK) return false;
-    double f = (b.bp == b.bq) ? 0.5 : 1.0;
-    if (kr == ks) f *= 0.5;
-    if (b.sb == t.sk) f *= 0.5;
-    t.f = f;
+    t.halvings = (b.bp == b.bq) + (kr == ks) + (b.sb == t.sk);
+    t.f = t.halvings == 0 ? 1.0 : t.halvings == 1 ? 0.5 : t.halvings == 2 ? 0.25 : 0.125;
     const int akr = hi16(r0.z), aks = lo16(r0.z);
     t.fr = hi16(r0.w);
     t.fs = lo16(r0.w);
@@ -320,6 +324,15 @@ __host__ __device__ constexpr int jk_min_waves(int lp, int lq, int lr, int ls)
     return lp + lq + lr + ls == 0 ? 8 : 1;
 }
 
+// Orders LDS accesses of the lanes of one wavefront (LDS executes a wavefront's
+// instructions in issue order; this keeps the compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ----------------------------------------------------------------------------------
 // The J/K kernel for one light class pair (LP LQ | LR LS): one lane per quartet
 // ----------------------------------------------------------------------------------
@@ -332,6 +345,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // the n-root Rys table in LDS (<= 18 KB): the root/weight lookups of every primitive
     // quartet then hit LDS instead of L2
     __shared__ __attribute__((aligned(16))) double rys_lds[rys_block_size(NRT)];
+#if TGPU_COMPACT
+    // per-wavefront queue of kets that passed the screening tests: (j, flags, fr, fs)
+    __shared__ int4 kqueue[4][128];
+#endif
     for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
         rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
     __syncthreads();
@@ -363,10 +380,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
             Jpq[p * NQ + q] = 0.0;
         }
     unsigned long long nq = 0, npq = 0, n32 = 0;
-    for (int j0 = 0; j0 < b.jend; j0 += 64) {
-        const int j = j0 + lane;
+    // one contracted quartet (bra pair, ket j of the prefix) that passed the tests
+    auto quartet = [&](const int j, const int flags, const int fr, const int fs) {
         KetTest t;
-        if (j >= b.jend || !test_ket(a, b, j, t)) continue;
+        t.sk = b.kb + j;
+        t.skipJ = flags & 1;
+        t.skipK = flags & 2;
+        t.skip_lists = flags & 4;
+        t.f = (flags >> 3) == 0 ? 1.0 : (flags >> 3) == 1 ? 0.5 : (flags >> 3) == 2 ? 0.25 : 0.125;
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         double I[NI];
@@ -379,7 +400,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
         const double f = t.f;
-        const int fr = t.fr, fs = t.fs;
         if (!t.skipJ) {
             double Prs[NRS];
 #pragma unroll
@@ -468,7 +488,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
                     unsafeAtomicAdd(&b.Y[(size_t)(fq + q) * ld + fs + s], f * acc);
                 }
         }
+    };
+    // Compaction: each round the 64 lanes test 64 kets of the prefix; the kets that pass
+    // are appended to this wavefront's queue in LDS, and the integral code runs only on
+    // full groups of 64 queued kets (all lanes busy), plus one partial group at the end.
+    // The queue is private to the wavefront, so ordering its LDS traffic needs no
+    // work-group barrier.
+#if TGPU_COMPACT
+    int4 *q = kqueue[threadIdx.x >> 6];
+    int qn = 0;  // wave-uniform
+    for (int j0 = 0;; j0 += 64) {
+        if (j0 < b.jend) {
+            const int j = j0 + lane;
+            KetTest t;
+            const bool ok = j < b.jend && test_ket(a, b, j, t);
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                q[pos] = make_int4(j, (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) |
+                                          (t.halvings << 3), t.fr, t.fs);
+            }
+            qn += __popcll(m);
+            wave_lds_sync();
+        }
+        const bool drain = j0 + 64 >= b.jend;  // nothing left to test
+        if (qn >= 64 || (drain && qn > 0)) {
+            // take the last min(qn, 64) entries; later rounds append behind the rest
+            const int take = qn < 64 ? qn : 64, base = qn - take;
+            const int4 e = lane < take ? q[base + lane] : make_int4(0, 0, 0, 0);
+            qn = base;
+            wave_lds_sync();
+            if (lane < take) quartet(e.x, e.y, e.z, e.w);
+        }
+        if (drain && qn == 0) break;
     }
+#else
+    for (int j0 = 0; j0 < b.jend; j0 += 64) {
+        const int j = j0 + lane;
+        KetTest t;
+        if (j < b.jend && test_ket(a, b, j, t))
+            quartet(j, (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) | (t.halvings << 3),
+                    t.fr, t.fs);
+    }
+#endif
     // wavefront reduction of the bra-pair J block
 #pragma unroll
     for (int pq = 0; pq < NPQ; ++pq) {

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtonto_gpu.so")
+# the library built in-tree by tonto_amd.build; TONTO_GPU_LIB selects another build of it
+LIB_PATH = os.environ.get("TONTO_GPU_LIB") or os.path.join(HERE, "libtonto_gpu.so")
 
 TGPU_JK_MIXED = 1
 TGPU_JK_GRAPH = 2
