@@ -99,7 +99,8 @@ def gen_class(LP, LQ, LR, LS, R):
     em(f"#pragma unroll", 2)
     em(f"for (int k = 0; k < {NE * NF}; ++k) G[k] = ({R})0;", 2)
     # outer loop over the ket's primitives (per-lane data, read once each), inner loop over
-    # the bra's primitives (the same for the whole wavefront, so those loads hit one line)
+    # the bra's primitives (the same for the whole wavefront when the caller's bra view
+    # is uniform: ldc() then makes them scalar loads)
     em("for (int iq = 0; iq < ket.n; ++iq) {", 2)
     em("const int iqq = ket.off + iq;", 3)
     em("const double zc = pm.zeta[iqq], izc = pm.zinv[iqq], Kc = pm.K[iqq];", 3)
@@ -107,8 +108,8 @@ def gen_class(LP, LQ, LR, LS, R):
     em(f"const {R} QCx = ({R})(Qx - ket.Ax), QCy = ({R})(Qy - ket.Ay), QCz = ({R})(Qz - ket.Az);", 3)
     em("for (int ip = 0; ip < bra.n; ++ip) {", 3)
     em("const int ipp = bra.off + ip;", 4)
-    em("const double za = pm.zeta[ipp], iza = pm.zinv[ipp], Ka = pm.K[ipp];", 4)
-    em("const double Px = pm.Px[ipp], Py = pm.Py[ipp], Pz = pm.Pz[ipp];", 4)
+    em("const double za = ldc(pm.zeta + ipp), iza = ldc(pm.zinv + ipp), Ka = ldc(pm.K + ipp);", 4)
+    em("const double Px = ldc(pm.Px + ipp), Py = ldc(pm.Py + ipp), Pz = ldc(pm.Pz + ipp);", 4)
     em(f"const {R} PAx = ({R})(Px - bra.Ax), PAy = ({R})(Py - bra.Ay), PAz = ({R})(Pz - bra.Az);", 4)
     em("const double izs = 1.0 / (za + zc), rho = za * zc * izs;", 4)
     em("const double QPx = Qx - Px, QPy = Qy - Py, QPz = Qz - Pz;", 4)

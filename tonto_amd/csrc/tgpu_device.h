@@ -18,6 +18,19 @@ struct DevPrims {
     const double *zeta, *zinv, *K, *Px, *Py, *Pz;
 };
 
+// Load from memory that no kernel of the build writes (pair and primitive tables) through
+// the constant address space: a wave-uniform address then becomes a scalar load into
+// SGPRs, a per-lane address stays an ordinary vector load.
+template <typename T>
+__device__ __forceinline__ T ldc(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T *)p;
+#else
+    return *p;  // the host pass never runs device functions
+#endif
+}
+
 // One shell pair as the integral code sees it: a run of n primitive pairs starting at
 // off in the primitive arrays (DevPrims), and the pair's A and A - B.  A 32-bit offset
 // rather than six pointers keeps the per-lane state small.
@@ -182,16 +195,16 @@ __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims
                                               bool skip)
 {
     PairView v;
-    const int4 r1 = pr.rec1[s];
+    const int4 r1 = ldc(pr.rec1 + s);
     const int off = skip ? r1.y : r1.x;
     v.n = skip ? lo16(r1.z) : hi16(r1.z);
     v.off = off;
-    v.Ax = pr.A[3 * s];
-    v.Ay = pr.A[3 * s + 1];
-    v.Az = pr.A[3 * s + 2];
-    v.ABx = pr.AB[3 * s];
-    v.ABy = pr.AB[3 * s + 1];
-    v.ABz = pr.AB[3 * s + 2];
+    v.Ax = ldc(pr.A + 3 * s);
+    v.Ay = ldc(pr.A + 3 * s + 1);
+    v.Az = ldc(pr.A + 3 * s + 2);
+    v.ABx = ldc(pr.AB + 3 * s);
+    v.ABy = ldc(pr.AB + 3 * s + 1);
+    v.ABz = ldc(pr.AB + 3 * s + 2);
     return v;
 }
 
@@ -244,35 +257,35 @@ struct BraInfo {
 __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
 {
     b.sb = a.bra_start + i;
-    const int4 r0 = a.pr.rec0[b.sb];
-    const int m = a.pr.rec1[b.sb].w;
-    b.kb = a.ml.seg_start[m * TGPU_NBIN + a.ket_bin];
-    const int kn = a.ml.seg_count[m * TGPU_NBIN + a.ket_bin];
-    const int jlim = a.same_bin ? b.sb - a.ml.seg_start[m * TGPU_NBIN + a.bra_bin] + 1 : kn;
-    const long long mo = a.ml.mat_off[m];
-    b.ld = a.ml.nbf[m];
+    const int4 r0 = ldc(a.pr.rec0 + b.sb);
+    const int m = ldc(a.pr.rec1 + b.sb).w;
+    b.kb = ldc(a.ml.seg_start + m * TGPU_NBIN + a.ket_bin);
+    const int kn = ldc(a.ml.seg_count + m * TGPU_NBIN + a.ket_bin);
+    const int jlim = a.same_bin ? b.sb - ldc(a.ml.seg_start + m * TGPU_NBIN + a.bra_bin) + 1 : kn;
+    const long long mo = ldc(a.ml.mat_off + m);
+    b.ld = ldc(a.ml.nbf + m);
     b.P = a.P + mo;
     b.X = a.X + mo;
     b.Y = a.Y + mo;
-    b.Pmax = a.Pmax + a.ml.pair_off[m];
-    const double P_max = a.scal[2 * m];
-    b.IP_max = a.scal[2 * m + 1];
+    b.Pmax = a.Pmax + ldc(a.ml.pair_off + m);
+    const double P_max = ldc(a.scal + 2 * m);
+    b.IP_max = ldc(a.scal + 2 * m + 1);
     b.tb = r0.x;
-    b.mIb = a.pr.maxI[b.sb];
+    b.mIb = ldc(a.pr.maxI + b.sb);
     b.bp = hi16(r0.y);
     b.bq = lo16(r0.y);
     b.atp = hi16(r0.z);
     b.atq = lo16(r0.z);
     b.fp = hi16(r0.w);
     b.fq = lo16(r0.w);
-    b.Pmax_b = b.Pmax[b.tb];
+    b.Pmax_b = ldc(b.Pmax + b.tb);
     b.maxI = a.pr.maxI + b.kb;
     // kets are sorted by max_I descending, so those passing
     // max_I(ab)*max_I(cd)*P_max >= cutoff (molecule.fock.foo:1430) form a prefix
     int lo = 0, hi = jlim;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (b.mIb * b.maxI[mid] * P_max < a.cutoff) hi = mid;
+        if (b.mIb * ldc(b.maxI + mid) * P_max < a.cutoff) hi = mid;
         else lo = mid + 1;
     }
     b.jend = lo;
@@ -291,17 +304,17 @@ struct KetTest {
 __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int j, KetTest &t)
 {
     t.sk = b.kb + j;
-    const int4 r0 = a.pr.rec0[t.sk];
+    const int4 r0 = ldc(a.pr.rec0 + t.sk);
     const int tk = r0.x;
-    const double mIk = b.maxI[j];
+    const double mIk = ldc(b.maxI + j);
     const bool bhi = b.tb >= tk;  // Tonto's (ab| is the pair with the larger index
     const double Iab = bhi ? b.mIb : mIk, Icd = bhi ? mIk : b.mIb;
     if (Iab * b.IP_max < a.cutoff) return false;
     const int kr = hi16(r0.y), ks = lo16(r0.y);
-    const double Pmax_k = b.Pmax[tk];
+    const double Pmax_k = ldc(b.Pmax + tk);
     const double JP = bhi ? fmax(b.Pmax_b, Pmax_k) : fmax(Pmax_k, b.Pmax_b);
-    const double KP = fmax(fmax(b.Pmax[tonto_pair(b.bp, kr)], b.Pmax[tonto_pair(b.bp, ks)]),
-                           fmax(b.Pmax[tonto_pair(b.bq, kr)], b.Pmax[tonto_pair(b.bq, ks)]));
+    const double KP = fmax(fmax(ldc(b.Pmax + tonto_pair(b.bp, kr)), ldc(b.Pmax + tonto_pair(b.bp, ks))),
+                           fmax(ldc(b.Pmax + tonto_pair(b.bq, kr)), ldc(b.Pmax + tonto_pair(b.bq, ks))));
     t.skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
     t.skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
     if (t.skipJ && t.skipK) return false;
@@ -317,11 +330,15 @@ __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int 
 }
 
 // Occupancy floor handed to the register allocator: (ss|ss) fits 8 wavefronts per SIMD
-// (<= 64 VGPRs) without spilling; every other class spills when pushed above what it
+// and (pp|ss) 4 without spilling; every other class spills when pushed above what it
 // reaches unconstrained, so it is left free.
 __host__ __device__ constexpr int jk_min_waves(int lp, int lq, int lr, int ls)
 {
-    return lp + lq + lr + ls == 0 ? 8 : 1;
+#ifdef TGPU_JK_WAVES_OVERRIDE
+    return TGPU_JK_WAVES_OVERRIDE;
+#else
+    return lp + lq + lr + ls == 0 ? 8 : (lp == 1 && lq == 1 && lr == 0 && ls == 0) ? 4 : 1;
+#endif
 }
 
 // Orders LDS accesses of the lanes of one wavefront (LDS executes a wavefront's
@@ -346,8 +363,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // quartet then hit LDS instead of L2
     __shared__ __attribute__((aligned(16))) double rys_lds[rys_block_size(NRT)];
 #if TGPU_COMPACT
-    // per-wavefront queue of kets that passed the screening tests: (j, flags, fr, fs)
-    __shared__ int4 kqueue[4][128];
+    // per-wavefront queues (full / skipped primitive lists) of kets that passed the
+    // screening tests: (j << 5 | flags, fr << 16 | fs); basis functions < 65536 (tgpu_create)
+    __shared__ int2 kqueue[4][2][128];
 #endif
     for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
         rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
@@ -366,7 +384,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         int lo = 0, hi = b.jend;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (b.mIb * b.maxI[mid] < a.fp32_thr) hi = mid;
+            if (b.mIb * ldc(b.maxI + mid) < a.fp32_thr) hi = mid;
             else lo = mid + 1;
         }
         jsplit = lo;
@@ -381,14 +399,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         }
     unsigned long long nq = 0, npq = 0, n32 = 0;
     // one contracted quartet (bra pair, ket j of the prefix) that passed the tests
-    auto quartet = [&](const int j, const int flags, const int fr, const int fs) {
+    // bv: the bra pair's view for this quartet's primitive lists (wave-uniform when the
+    // caller's list choice is)
+    auto quartet = [&](const int j, const int flags, const int fr, const int fs, const PairView &bv) {
         KetTest t;
         t.sk = b.kb + j;
         t.skipJ = flags & 1;
         t.skipK = flags & 2;
         t.skip_lists = flags & 4;
         t.f = (flags >> 3) == 0 ? 1.0 : (flags >> 3) == 1 ? 0.5 : (flags >> 3) == 2 ? 0.25 : 0.125;
-        const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         double I[NI];
         if (!MIXED || j < jsplit) {
@@ -495,33 +514,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // The queue is private to the wavefront, so ordering its LDS traffic needs no
     // work-group barrier.
 #if TGPU_COMPACT
-    int4 *q = kqueue[threadIdx.x >> 6];
-    int qn = 0;  // wave-uniform
+    // Two queues per wavefront: kets whose quartet uses the full primitive lists (two of
+    // the four atoms coincide) and kets using the skipped lists.  A group of
+    // 64 is always drawn from one queue, so the bra view is wave-uniform and its primitive
+    // data is read with scalar loads.
+    int2(*q)[128] = kqueue[threadIdx.x >> 6];
+    int qn0 = 0, qn1 = 0;  // wave-uniform
     for (int j0 = 0;; j0 += 64) {
         if (j0 < b.jend) {
             const int j = j0 + lane;
             KetTest t;
             const bool ok = j < b.jend && test_ket(a, b, j, t);
-            const unsigned long long m = __ballot(ok);
+            const unsigned long long m1 = __ballot(ok && t.skip_lists), m0 = __ballot(ok && !t.skip_lists);
             if (ok) {
-                const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                q[pos] = make_int4(j, (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) |
-                                          (t.halvings << 3), t.fr, t.fs);
+                const unsigned long long m = t.skip_lists ? m1 : m0;
+                const int pos = (t.skip_lists ? qn1 : qn0) +
+                                (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                q[t.skip_lists ? 1 : 0][pos] =
+                    make_int2((j << 5) | (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) |
+                                  (t.halvings << 3),
+                              (t.fr << 16) | t.fs);
             }
-            qn += __popcll(m);
+            qn0 += __popcll(m0);
+            qn1 += __popcll(m1);
             wave_lds_sync();
         }
         const bool drain = j0 + 64 >= b.jend;  // nothing left to test
-        if (qn >= 64 || (drain && qn > 0)) {
-            // take the last min(qn, 64) entries; later rounds append behind the rest
+        for (;;) {
+            // a full group from either queue, or (draining) what is left
+            const int L = qn1 >= 64 ? 1 : qn0 >= 64 ? 0 : !drain ? -1 : qn1 > 0 ? 1 : qn0 > 0 ? 0 : -1;
+            if (L < 0) break;
+            int &qn = L ? qn1 : qn0;
             const int take = qn < 64 ? qn : 64, base = qn - take;
-            const int4 e = lane < take ? q[base + lane] : make_int4(0, 0, 0, 0);
+            const int2 e = lane < take ? q[L][base + lane] : make_int2(0, 0);
             qn = base;
             wave_lds_sync();
-            if (lane < take) quartet(e.x, e.y, e.z, e.w);
+            const PairView bv = make_view(a.pr, a.pm, b.sb, L == 1);  // wave-uniform
+            if (lane < take) quartet(e.x >> 5, e.x & 31, e.y >> 16, e.y & 0xffff, bv);
         }
-        if (drain && qn == 0) break;
+        if (drain) break;
     }
 #else
     for (int j0 = 0; j0 < b.jend; j0 += 64) {
@@ -529,7 +561,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         KetTest t;
         if (j < b.jend && test_ket(a, b, j, t))
             quartet(j, (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) | (t.halvings << 3),
-                    t.fr, t.fs);
+                    t.fr, t.fs, make_view(a.pr, a.pm, b.sb, t.skip_lists));
     }
 #endif
     // wavefront reduction of the bra-pair J block
@@ -656,13 +688,13 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
     const int ax = lane / N, rr = lane - (lane / N) * N;
     for (int ip = 0; ip < bra.n; ++ip) {
         const int ipp = bra.off + ip;
-        const double za = pm.zeta[ipp], Ka = pm.K[ipp];
-        const double Px = pm.Px[ipp], Py = pm.Py[ipp], Pz = pm.Pz[ipp];
+        const double za = ldc(pm.zeta + ipp), Ka = ldc(pm.K + ipp);
+        const double Px = ldc(pm.Px + ipp), Py = ldc(pm.Py + ipp), Pz = ldc(pm.Pz + ipp);
         for (int iq = 0; iq < ket.n; ++iq) {
             if (lane < 3 * N) {
                 const int iqq = ket.off + iq;
-                const double zc = pm.zeta[iqq], Kc = pm.K[iqq];
-                const double Qx = pm.Px[iqq], Qy = pm.Py[iqq], Qz = pm.Pz[iqq];
+                const double zc = ldc(pm.zeta + iqq), Kc = ldc(pm.K + iqq);
+                const double Qx = ldc(pm.Px + iqq), Qy = ldc(pm.Py + iqq), Qz = ldc(pm.Pz + iqq);
                 const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
                 const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
                 const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
