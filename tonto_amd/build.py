@@ -17,6 +17,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+# another copy of the kernel sources (variant builds)
+CSRC = os.environ.get("TONTO_GPU_CSRC") or CSRC
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtonto_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -22,9 +22,20 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
 {
 #if !TGPU_HEAVY
     if (variant == 0) {
+        // the LDS-row form when the bra pair's K rows fit a 64 KiB work-group next to the
+        // Rys table and the ket queues
+        constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
+        constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
+        constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 2 * 128 * 8 : 0);
+        const size_t rows = (size_t)(NP + NQ) * a.max_nbf * sizeof(double);
+        if (a.max_nbf > 0 && rows + kStatic <= 64 * 1024) {
+            if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(a);
+            else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(a);
+            return;
+        }
         const int blocks = (a.n_bra + 3) / 4;
-        if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true><<<blocks, 256, 0, s>>>(a);
-        else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false><<<blocks, 256, 0, s>>>(a);
+        if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, false><<<blocks, 256, 0, s>>>(a);
+        else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, false><<<blocks, 256, 0, s>>>(a);
         return;
     }
 #endif

@@ -240,6 +240,7 @@ struct JKArgs {
     int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot, ket_chunk, n_ket_max;
     double *ws;    // wavefront-form workspace: ws_slots slots of the class's WaveCfg::WS doubles
     int ws_slots;
+    int max_nbf;   // largest molecule (basis functions): the LDS rows of the KROWS form
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -353,7 +354,12 @@ __device__ __forceinline__ void wave_lds_sync()
 // ----------------------------------------------------------------------------------
 // The J/K kernel for one light class pair (LP LQ | LR LS): one lane per quartet
 // ----------------------------------------------------------------------------------
-template <int LP, int LQ, int LR, int LS, bool MIXED>
+// KROWS: one bra pair per work-group, its four wavefronts taking interleaved rounds of the
+// ket prefix; the K updates of the bra pair (all in its NP + NQ rows) accumulate in LDS
+// (dynamic shared memory, (NP + NQ) x n_bf doubles) and are flushed once, row by row,
+// with coalesced atomics.  Otherwise four bra pairs per work-group, one per wavefront,
+// and every K update is a global atomic.
+template <int LP, int LQ, int LR, int LS, bool MIXED, bool KROWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves(LP, LQ, LR, LS), 8))) void jk_kernel(JKArgs a)
 {
     constexpr int NP = ncart(LP), NQ = ncart(LQ), NR = ncart(LR), NS = ncart(LS);
@@ -367,17 +373,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // screening tests: (j << 5 | flags, fr << 16 | fs); basis functions < 65536 (tgpu_create)
     __shared__ int2 kqueue[4][2][128];
 #endif
+    extern __shared__ double krow[];  // KROWS: [NP + NQ][ld]
     for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
         rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
-    __syncthreads();
     const int lane = threadIdx.x & 63;
     // the bra pair is uniform over the wavefront: readfirstlane lets the compiler keep
     // everything derived from it in scalar registers
-    const int i = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = KROWS ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave;
+    if (KROWS && i >= a.n_bra) return;  // uniform over the work-group
+    __syncthreads();
     if (i >= a.n_bra) return;
     BraInfo b;
     load_bra(a, i, b);
     const int ld = b.ld, fp = b.fp, fq = b.fq;
+    if (KROWS) {
+        for (int t = threadIdx.x; t < (NP + NQ) * ld; t += blockDim.x) krow[t] = 0.0;
+        __syncthreads();
+    }
+    // K(row, col) += v: row = bra function fp + p (p < NP) or fq + q (NP + q)
+    auto kadd = [&](const int row, const int col, const double v) {
+        if (KROWS) {
+            __hip_atomic_fetch_add(&krow[row * ld + col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);
+        }
+    };
     const double *__restrict__ P = b.P;
     int jsplit = b.jend;
     if (MIXED) {
@@ -468,7 +489,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Psq[s * NQ + q];
-                    unsafeAtomicAdd(&b.Y[(size_t)(fp + p) * ld + fr + r], f * acc);
+                    kadd(p, fr + r, f * acc);
                 }
 #pragma unroll
             for (int p = 0; p < NP; ++p)
@@ -480,7 +501,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Prq[r * NQ + q];
-                    unsafeAtomicAdd(&b.Y[(size_t)(fp + p) * ld + fs + s], f * acc);
+                    kadd(p, fs + s, f * acc);
                 }
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -492,7 +513,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Psp[s * NP + p];
-                    unsafeAtomicAdd(&b.Y[(size_t)(fq + q) * ld + fr + r], f * acc);
+                    kadd(NP + q, fr + r, f * acc);
                 }
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Prp[r * NP + p];
-                    unsafeAtomicAdd(&b.Y[(size_t)(fq + q) * ld + fs + s], f * acc);
+                    kadd(NP + q, fs + s, f * acc);
                 }
         }
     };
@@ -520,7 +541,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // data is read with scalar loads.
     int2(*q)[128] = kqueue[threadIdx.x >> 6];
     int qn0 = 0, qn1 = 0;  // wave-uniform
-    for (int j0 = 0;; j0 += 64) {
+    constexpr int step = KROWS ? 256 : 64;  // KROWS: the four wavefronts interleave rounds
+    for (int j0 = KROWS ? 64 * wave : 0;; j0 += step) {
         if (j0 < b.jend) {
             const int j = j0 + lane;
             KetTest t;
@@ -540,7 +562,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
             qn1 += __popcll(m1);
             wave_lds_sync();
         }
-        const bool drain = j0 + 64 >= b.jend;  // nothing left to test
+        const bool drain = j0 + step >= b.jend;  // nothing left to test
         for (;;) {
             // a full group from either queue, or (draining) what is left
             const int L = qn1 >= 64 ? 1 : qn0 >= 64 ? 0 : !drain ? -1 : qn1 > 0 ? 1 : qn0 > 0 ? 0 : -1;
@@ -556,7 +578,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         if (drain) break;
     }
 #else
-    for (int j0 = 0; j0 < b.jend; j0 += 64) {
+    for (int j0 = KROWS ? 64 * wave : 0; j0 < b.jend; j0 += KROWS ? 256 : 64) {
         const int j = j0 + lane;
         KetTest t;
         if (j < b.jend && test_ket(a, b, j, t))
@@ -583,6 +605,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         atomicAdd(&a.counters[3 * a.counter_slot + 0], nq);
         atomicAdd(&a.counters[3 * a.counter_slot + 1], npq);
         atomicAdd(&a.counters[3 * a.counter_slot + 2], n32);
+    }
+    if (KROWS) {
+        // every wavefront of the work-group is here (one bra pair, no early return)
+        __syncthreads();
+        for (int t = threadIdx.x; t < (NP + NQ) * ld; t += blockDim.x) {
+            const double v = krow[t];
+            const int row = t / ld, col = t - row * ld;
+            if (v != 0.0) unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);
+        }
     }
 }
 

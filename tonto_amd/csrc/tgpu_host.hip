@@ -868,6 +868,7 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.counters = c->d_counters.p;
     a.ws = c->d_ws.p;
     a.ws_slots = c->ws_slots;
+    a.max_nbf = *std::max_element(c->nbf.begin(), c->nbf.end());
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];
         const bool prof = timing && c->profiling;
