@@ -22,9 +22,10 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
 {
 #if !TGPU_HEAVY
     if (variant == 0) {
-        // the LDS-row form when the bra pair's K rows fit a 64 KiB work-group next to the
-        // Rys table and the ket queues
+        // the LDS-row form for (ss| and (ps| bra pairs (2 and 4 K rows) when the rows fit a
+        // 64 KiB work-group next to the Rys table and the ket queues
         constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
+#if TGPU_LP + TGPU_LQ <= 1
         constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
         constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 2 * 128 * 8 : 0);
         const size_t rows = (size_t)(NP + NQ) * a.max_nbf * sizeof(double);
@@ -33,6 +34,10 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(a);
             return;
         }
+#else
+        (void)NP;
+        (void)NQ;
+#endif
         const int blocks = (a.n_bra + 3) / 4;
         if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, false><<<blocks, 256, 0, s>>>(a);
         else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, false><<<blocks, 256, 0, s>>>(a);
