@@ -22,14 +22,25 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
 {
 #if !TGPU_HEAVY
     if (variant == 0) {
-        // the LDS-row form for (ss| and (ps| bra pairs (2 and 4 K rows) when the rows fit a
-        // 64 KiB work-group next to the Rys table and the ket queues
+        // the LDS-row form for (ss|, (ps| bra pairs, and (pp|, (ds| against (ss| and (ps|
+        // kets (2-7 K rows, kernels well below the register limit), when the rows fit a
+        // 96 KiB work-group next to the Rys table and the ket queues
         constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
-#if TGPU_LP + TGPU_LQ <= 1
+#if (TGPU_LP + TGPU_LQ <= 1) || (TGPU_LP + TGPU_LQ == 2 && TGPU_LR + TGPU_LS <= 1)
         constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
         constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 2 * 128 * 8 : 0);
+        constexpr size_t kBudget = 96 * 1024;
+        static bool attr_set = false;
+        if (!attr_set) {
+            // dynamic LDS above the 64 KiB default has to be allowed per kernel
+            (void)hipFuncSetAttribute((const void *)jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kBudget - kStatic));
+            (void)hipFuncSetAttribute((const void *)jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kBudget - kStatic));
+            attr_set = true;
+        }
         const size_t rows = (size_t)(NP + NQ) * a.max_nbf * sizeof(double);
-        if (a.max_nbf > 0 && rows + kStatic <= 64 * 1024) {
+        if (a.max_nbf > 0 && rows + kStatic <= kBudget) {
             if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(a);
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(a);
             return;
