@@ -63,7 +63,7 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
     // grid holds about kWaveGrid work-groups; workspace classes stay within ws_slots.
     constexpr long long kWaveGrid = 65536;
     if (a.n_bra <= 0) return;
-    const int kc = a.ket_chunk > 0 ? a.ket_chunk : 16;
+    const int kc = a.ket_chunk > 0 && a.ket_chunk <= 64 ? a.ket_chunk : 64;  // one lane per ket
     const long long nch = a.n_ket_max > 0 ? (a.n_ket_max + kc - 1) / kc : 1;
     long long gx = a.n_bra, gy;
     if (C::GB) {

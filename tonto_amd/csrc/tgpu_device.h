@@ -701,16 +701,15 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
         (void)gbuf;
     }
     double Gr[MG];
-    int gi[MG][3];
+    // (e0|f0) entry k reads the x, y, z tables at gidx[3k..3k+2] (< TW <= 64): packed 6 bits each
+    int gi[MG];
     if constexpr (C::GREG) {
 #pragma unroll
         for (int m = 0; m < MG; ++m) {
             Gr[m] = 0.0;
             const int k = lane + 64 * m;
             const int kk = k < NG ? k : 0;
-            gi[m][0] = T::gidx[3 * kk];
-            gi[m][1] = TW * N + T::gidx[3 * kk + 1];
-            gi[m][2] = 2 * TW * N + T::gidx[3 * kk + 2];
+            gi[m] = T::gidx[3 * kk] | (T::gidx[3 * kk + 1] << 6) | (T::gidx[3 * kk + 2] << 12);
         }
     } else {
         // each lane owns entries lane, lane+64, ... until the transfer step
@@ -737,31 +736,37 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
                 const double rz = ur * rho / za, re = ur * rho / zc;
                 const double b10 = (1.0 - rz) * 0.5 / za, b01 = (1.0 - re) * 0.5 / zc, b00 = ur * 0.5 * izs;
                 const double c00 = PA + rz * qp, d00 = QC - re * qp;
-                double t[W1][W2];
-                t[0][0] = ax == 2 ? wr * TWOPI5ON2 * Ka * Kc * sqrt(rho) : 1.0;
+                // the 2-D table I(e, f) column by column (two columns live), each column
+                // stored to LDS as soon as it is complete
+                double *dst = L.g + (ax * N + rr) * TW;
+                double cp[W1], cc[W1];
+                cc[0] = ax == 2 ? wr * TWOPI5ON2 * Ka * Kc * sqrt(rho) : 1.0;
 #pragma unroll
-                for (int e = 0; e + 1 < W1; ++e)
-                    t[e + 1][0] = c00 * t[e][0] + (e ? e * b10 * t[e - 1][0] : 0.0);
+                for (int e = 0; e + 1 < W1; ++e) cc[e + 1] = c00 * cc[e] + (e ? e * b10 * cc[e - 1] : 0.0);
 #pragma unroll
-                for (int f = 0; f + 1 < W2; ++f)
+                for (int e = 0; e < W1; ++e) dst[e * W2] = cc[e];
+#pragma unroll
+                for (int f = 0; f + 1 < W2; ++f) {
+                    double cn[W1];
 #pragma unroll
                     for (int e = 0; e < W1; ++e)
-                        t[e][f + 1] = d00 * t[e][f] + (f ? f * b01 * t[e][f - 1] : 0.0) +
-                                      (e ? e * b00 * t[e - 1][f] : 0.0);
-                double *dst = L.g + (ax * N + rr) * TW;
+                        cn[e] = d00 * cc[e] + (f ? f * b01 * cp[e] : 0.0) + (e ? e * b00 * cc[e - 1] : 0.0);
 #pragma unroll
-                for (int e = 0; e < W1; ++e)
-#pragma unroll
-                    for (int f = 0; f < W2; ++f) dst[e * W2 + f] = t[e][f];
+                    for (int e = 0; e < W1; ++e) {
+                        dst[e * W2 + f + 1] = cn[e];
+                        cp[e] = cc[e];
+                        cc[e] = cn[e];
+                    }
+                }
             }
             __syncthreads();
             if constexpr (C::GREG) {
 #pragma unroll
                 for (int m = 0; m < MG; ++m) {
+                    const int i0 = gi[m] & 63, i1 = TW * N + ((gi[m] >> 6) & 63), i2 = 2 * TW * N + (gi[m] >> 12);
                     double acc = 0.0;
 #pragma unroll
-                    for (int r = 0; r < N; ++r)
-                        acc += L.g[gi[m][0] + r * TW] * L.g[gi[m][1] + r * TW] * L.g[gi[m][2] + r * TW];
+                    for (int r = 0; r < N; ++r) acc += L.g[i0 + r * TW] * L.g[i1 + r * TW] * L.g[i2 + r * TW];
                     Gr[m] += acc;
                 }
             } else {
@@ -832,14 +837,33 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
 #pragma unroll
     for (int m = 0; m < MJ; ++m) Jpq[m] = 0.0;
     unsigned long long nq = 0, npq = 0;
-    for (int jc = blockIdx.y * a.ket_chunk; jc < b.jend; jc += step)
-    for (int j = jc; j < min(b.jend, jc + a.ket_chunk); ++j) {  // uniform over the work-group
+    // the 64 lanes screen up to 64 kets of a chunk at once; the kets that pass are then
+    // taken one after the other by the whole wavefront (their test results broadcast from
+    // the lane that computed them)
+    for (int jc = blockIdx.y * a.ket_chunk; jc < b.jend; jc += step) {
+        const int jlim = min(b.jend, jc + a.ket_chunk);
+        KetTest tl;
+        const bool ok = jc + lane < jlim && test_ket(a, b, jc + lane, tl);
+        const int flags_l = ok ? ((int)tl.skipJ | ((int)tl.skipK << 1) | ((int)tl.skip_lists << 2) |
+                                  (tl.halvings << 3))
+                               : 0;
+        const int frs_l = ok ? (tl.fr << 16) | tl.fs : 0;
+        unsigned long long pass = __ballot(ok);
+    while (pass) {  // uniform over the work-group
+        const int l = __builtin_ctzll(pass);
+        pass &= pass - 1;
+        const int j = jc + l;
+        const int flags = __builtin_amdgcn_readlane(flags_l, l), frs = __builtin_amdgcn_readlane(frs_l, l);
         KetTest t;
-        if (!test_ket(a, b, j, t)) continue;
+        t.sk = b.kb + j;
+        t.skipJ = flags & 1;
+        t.skipK = flags & 2;
+        t.skip_lists = flags & 4;
+        t.f = (flags >> 3) == 0 ? 1.0 : (flags >> 3) == 1 ? 0.5 : (flags >> 3) == 2 ? 0.25 : 0.125;
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         const double f = t.f;
-        const int fr = t.fr, fs = t.fs;
+        const int fr = frs >> 16, fs = frs & 0xffff;
         // stage the ket-dependent density blocks
         for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
         for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fs + u / NQ) * ld + fq + u % NQ];
@@ -852,6 +876,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         if (!t.skipJ) {
             for (int u = lane; u < NRS; u += 64) {
                 double acc = 0.0;
+#pragma unroll 4
                 for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + u] * L.Ppq[pq];
                 unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
             }
@@ -860,6 +885,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
                 const int pq = lane + 64 * m;
                 if (pq < NPQ) {
                     double acc = 0.0;
+#pragma unroll 4
                     for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * L.Prs[rs];
                     Jpq[m] += f * acc;
                 }
@@ -872,22 +898,30 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
                 size_t dst;
                 if (u < N1) {  // K(p,r) += I P(s,q)
                     const int p = u / NR, r = u % NR;
+#pragma unroll 1
                     for (int q = 0; q < NQ; ++q)
+#pragma unroll 4
                         for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psq[s * NQ + q];
                     dst = (size_t)(fp + p) * ld + fr + r;
                 } else if (u < N2) {  // K(p,s) += I P(r,q)
                     const int p = (u - N1) / NS, s = (u - N1) % NS;
+#pragma unroll 1
                     for (int q = 0; q < NQ; ++q)
+#pragma unroll 4
                         for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prq[r * NQ + q];
                     dst = (size_t)(fp + p) * ld + fs + s;
                 } else if (u < N3) {  // K(q,r) += I P(s,p)
                     const int q = (u - N2) / NR, r = (u - N2) % NR;
+#pragma unroll 1
                     for (int p = 0; p < NP; ++p)
+#pragma unroll 4
                         for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psp[s * NP + p];
                     dst = (size_t)(fq + q) * ld + fr + r;
                 } else {  // K(q,s) += I P(r,p)
                     const int q = (u - N3) / NS, s = (u - N3) % NS;
+#pragma unroll 1
                     for (int p = 0; p < NP; ++p)
+#pragma unroll 4
                         for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prp[r * NP + p];
                     dst = (size_t)(fq + q) * ld + fs + s;
                 }
@@ -895,6 +929,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
             }
         }
         __syncthreads();
+    }
     }
 #pragma unroll
     for (int m = 0; m < MJ; ++m) {

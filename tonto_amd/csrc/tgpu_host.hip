@@ -888,7 +888,12 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
                 a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
                 a.same_bin = X == Y;
                 a.n_ket_max = c->bin_max_seg[Y];
-                a.ket_chunk = 16;
+                // wavefront form: one lane screens each ket of a chunk; chunks of 64 unless
+                // that leaves fewer than ~128k work-groups, then 32 or 16
+                a.ket_chunk = 64;
+                while (a.ket_chunk > 16 &&
+                       (long long)a.n_bra * ((a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk) < 131072)
+                    a.ket_chunk /= 2;
                 if (a.n_bra == 0 || c->bin_start[Y + 1] == c->bin_start[Y]) continue;
                 kLaunch[k](a, c->variant[k], mixed, s);
             }
