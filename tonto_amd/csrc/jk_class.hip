@@ -60,16 +60,17 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
     using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS>;
     // work-group (x, y): bra pairs x, x + gx, ...; of each, ket chunks y, y + gy, ... of
     // a.ket_chunk kets.  gy splits long prefixes when there are few bra pairs, so that the
-    // grid holds about kWaveGrid work-groups; workspace classes stay within ws_slots.
+    // grid holds about kWaveGrid work-groups; workspace classes stay within their slots.
     constexpr long long kWaveGrid = 65536;
     if (a.n_bra <= 0) return;
     const int kc = a.ket_chunk > 0 && a.ket_chunk <= 64 ? a.ket_chunk : 64;  // one lane per ket
     const long long nch = a.n_ket_max > 0 ? (a.n_ket_max + kc - 1) / kc : 1;
     long long gx = a.n_bra, gy;
     if (C::GB) {
-        if (a.ws_slots <= 0 || !a.ws) return;  // unreachable: tgpu_schwarz allocates it first
-        gx = std::min<long long>(gx, a.ws_slots);
-        gy = std::min<long long>(nch, std::max<long long>(1, a.ws_slots / gx));
+        const long long slots = ws_slot_count(a.ws_doubles, C::WS);
+        if (slots <= 0 || !a.ws) return;  // unreachable: tgpu_schwarz allocates it first
+        gx = std::min<long long>(gx, slots);
+        gy = std::min<long long>(nch, std::max<long long>(1, slots / gx));
     } else {
         gy = std::min<long long>(nch, std::max<long long>(1, (kWaveGrid + gx - 1) / gx));
     }
@@ -86,15 +87,16 @@ size_t TGPU_WS_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)()
 
 #if TGPU_LP == TGPU_LR && TGPU_LQ == TGPU_LS
 void TGPU_SCHWARZ_NAME_X(TGPU_LP, TGPU_LQ)(const DevPairs &pr, const DevPrims &pm, int start, int n,
-                                          double *maxI, double *ws, int ws_slots, hipStream_t s)
+                                          double *maxI, double *ws, long long ws_doubles, hipStream_t s)
 {
 #if TGPU_HEAVY
     using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LP, TGPU_LQ>;
-    const int grid = C::GB ? (n < ws_slots ? n : ws_slots) : n;
+    const long long slots = ws_slot_count(ws_doubles, C::WS);
+    const int grid = C::GB ? (int)(n < slots ? n : slots) : n;
     if (grid > 0) schwarz_wave_kernel<TGPU_LP, TGPU_LQ><<<grid, 64, 0, s>>>(pr, pm, start, n, maxI, ws);
 #else
     (void)ws;
-    (void)ws_slots;
+    (void)ws_doubles;
     schwarz_kernel<TGPU_LP, TGPU_LQ><<<(n + 63) / 64, 64, 0, s>>>(pr, pm, start, n, maxI);
 #endif
 }

@@ -238,8 +238,8 @@ struct JKArgs {
     DevPrims pm;
     DevMols ml;
     int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot, ket_chunk, n_ket_max;
-    double *ws;    // wavefront-form workspace: ws_slots slots of the class's WaveCfg::WS doubles
-    int ws_slots;
+    double *ws;    // wavefront-form workspace of ws_doubles doubles: a class uses
+    long long ws_doubles;  // min(8192, ws_doubles / WaveCfg::WS) slots
     int max_nbf;   // largest molecule (basis functions): the LDS rows of the KROWS form
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
@@ -646,6 +646,12 @@ __global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, i
 // the bra-side transfer relation run level by level in LDS with lanes over entries;
 // the digestion gives each J/K target element to one lane (one atomic per element).
 // ----------------------------------------------------------------------------------
+// slots of a workspace class: one per resident work-group of its persistent grid
+__host__ __device__ constexpr long long ws_slot_count(long long ws_doubles, long long per_slot)
+{
+    return per_slot <= 0 ? 0 : (ws_doubles / per_slot < 8192 ? ws_doubles / per_slot : 8192);
+}
+
 // LDS budget of the wavefront form.  A class whose transfer buffers would take more keeps
 // them in a global workspace slot instead (one slot per resident work-group, persistent
 // grid), so that the 160 KiB of a CU still holds several work-groups.
@@ -945,7 +951,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
 }
 
 // Wavefront form: grid (gx, gy); work-group (x, y) takes bra pairs x, x + gx, ... and, of
-// each, the ket chunks y, y + gy, ...  Workspace classes keep gx * gy <= ws_slots and use
+// each, the ket chunks y, y + gy, ...  Workspace classes keep gx * gy <= their slot count and use
 // slot y * gx + x.
 template <int LP, int LQ, int LR, int LS>
 __global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
@@ -989,6 +995,6 @@ TGPU_FOR_EACH_CLASS(TGPU_DECLARE_JK)
 #undef TGPU_DECLARE_JK
 #define TGPU_DECLARE_SCHWARZ(LP, LQ, H) \
     void tgpu_launch_schwarz_##LP##LQ(const DevPairs &pr, const DevPrims &pm, int start, int n, double *maxI, \
-                                      double *ws, int ws_slots, hipStream_t s);
+                                      double *ws, long long ws_doubles, hipStream_t s);
 TGPU_FOR_EACH_PAIRCLASS(TGPU_DECLARE_SCHWARZ)
 #undef TGPU_DECLARE_SCHWARZ

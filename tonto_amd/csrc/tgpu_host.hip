@@ -306,7 +306,7 @@ struct tgpu_ctx {
     DBuf<unsigned long long> d_counters;
     // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
     DBuf<double> d_ws;
-    int ws_slots = 0;
+    long long ws_doubles = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
     bool timed = false;
     bool profiling = false;
@@ -759,11 +759,9 @@ static const int kClassPair[NQCLS][2] = {
 #undef TGPU_CLASSPAIR
 };
 
-// Work-groups (and workspace slots) of a workspace-class launch: 4 per CU of an MI355X
-static const int kWsSlots = 1024;
-
 // Allocates the wavefront-form workspace for the class pairs this basis has (none for
-// s/p/d bases): kWsSlots slots of the largest per-slot need.
+// s/p/d bases): room for at least 1024 slots of the largest per-slot need, and up to
+// 8192 slots or 1 GiB, so classes with smaller transfer buffers run wider grids.
 static int ensure_workspace(tgpu_ctx *c)
 {
     size_t need = 0;
@@ -774,8 +772,9 @@ static int ensure_workspace(tgpu_ctx *c)
         if (nb && nk) need = std::max(need, kWaveWs[k]());
     }
     if (!need) return 0;
-    if (int rc = dalloc(c, c->d_ws, need * kWsSlots)) return rc;
-    c->ws_slots = kWsSlots;
+    const size_t n = std::max(1024 * need, std::min(8192 * need, (size_t)(1 << 30) / sizeof(double)));
+    if (int rc = dalloc(c, c->d_ws, n)) return rc;
+    c->ws_doubles = (long long)n;
     return 0;
 }
 
@@ -790,7 +789,7 @@ extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
     {                                                                                    \
         const int b0 = pair_class(LP, LQ) * NBK, n = c->bin_start[b0 + NBK] - c->bin_start[b0]; \
         if (n) tgpu_launch_schwarz_##LP##LQ(dpr, dpm, c->bin_start[b0], n, c->d_maxI.p, c->d_ws.p, \
-                                            c->ws_slots, c->stream);                          \
+                                            c->ws_doubles, c->stream);                        \
     }
         TGPU_FOR_EACH_PAIRCLASS(TGPU_SCHWARZ_CASE)
 #undef TGPU_SCHWARZ_CASE
@@ -867,7 +866,7 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.Y = c->d_Y.p;
     a.counters = c->d_counters.p;
     a.ws = c->d_ws.p;
-    a.ws_slots = c->ws_slots;
+    a.ws_doubles = c->ws_doubles;
     a.max_nbf = *std::max_element(c->nbf.begin(), c->nbf.end());
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];
