@@ -24,11 +24,11 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
     if (variant == 0) {
         // the LDS-row form for (ss|, (ps| bra pairs, and (pp|, (ds| against (ss| and (ps|
         // kets (2-7 K rows, kernels well below the register limit), when the rows fit a
-        // 96 KiB work-group next to the Rys table and the ket queues
+        // 96 KiB work-group next to the Rys table and the ket queues,
         constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
 #if (TGPU_LP + TGPU_LQ <= 1) || (TGPU_LP + TGPU_LQ == 2 && TGPU_LR + TGPU_LS <= 1)
         constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
-        constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 2 * 128 * 8 : 0);
+        constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 128 * 8 : 0);
         constexpr size_t kBudget = 96 * 1024;
         static bool attr_set = false;
         if (!attr_set) {
@@ -40,7 +40,10 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
             attr_set = true;
         }
         const size_t rows = (size_t)(NP + NQ) * a.max_nbf * sizeof(double);
-        if (a.max_nbf > 0 && rows + kStatic <= kBudget) {
+        // and when the molecules' ket segments are on average long enough (>= 4 rounds of
+        // 64) to keep the four wavefronts of one bra pair busy: batches of small molecules
+        // keep four bra pairs per work-group
+        if (a.max_nbf > 0 && rows + kStatic <= kBudget && a.n_ket_avg >= 256) {
             if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(a);
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(a);
             return;

@@ -238,6 +238,7 @@ struct JKArgs {
     DevPrims pm;
     DevMols ml;
     int bra_start, n_bra, bra_bin, ket_bin, same_bin, counter_slot, ket_chunk, n_ket_max;
+    int n_ket_avg;  // ket-bin pairs of a single molecule (0 for batches)
     double *ws;    // wavefront-form workspace of ws_doubles doubles: a class uses
     long long ws_doubles;  // min(8192, ws_doubles / WaveCfg::WS) slots
     int max_nbf;   // largest molecule (basis functions): the LDS rows of the KROWS form
@@ -369,9 +370,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // quartet then hit LDS instead of L2
     __shared__ __attribute__((aligned(16))) double rys_lds[rys_block_size(NRT)];
 #if TGPU_COMPACT
-    // per-wavefront queues (full / skipped primitive lists) of kets that passed the
-    // screening tests: (j << 5 | flags, fr << 16 | fs); basis functions < 65536 (tgpu_create)
-    __shared__ int2 kqueue[4][2][128];
+    // per-wavefront queue of kets that passed the screening tests:
+    // (j << 5 | flags, fr << 16 | fs); basis functions < 65536 (tgpu_create)
+    __shared__ int2 kqueue[4][128];
 #endif
     extern __shared__ double krow[];  // KROWS: [NP + NQ][ld]
     for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
@@ -420,8 +421,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         }
     unsigned long long nq = 0, npq = 0, n32 = 0;
     // one contracted quartet (bra pair, ket j of the prefix) that passed the tests
-    // bv: the bra pair's view for this quartet's primitive lists (wave-uniform when the
-    // caller's list choice is)
+    // bv: the bra pair's view for this quartet's primitive lists
     auto quartet = [&](const int j, const int flags, const int fr, const int fs, const PairView &bv) {
         KetTest t;
         t.sk = b.kb + j;
@@ -535,47 +535,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // The queue is private to the wavefront, so ordering its LDS traffic needs no
     // work-group barrier.
 #if TGPU_COMPACT
-    // Two queues per wavefront: kets whose quartet uses the full primitive lists (two of
-    // the four atoms coincide) and kets using the skipped lists.  A group of
-    // 64 is always drawn from one queue, so the bra view is wave-uniform and its primitive
-    // data is read with scalar loads.
-    int2(*q)[128] = kqueue[threadIdx.x >> 6];
-    int qn0 = 0, qn1 = 0;  // wave-uniform
+    // Compaction: each round the 64 lanes test 64 kets of the prefix; the kets that pass
+    // are appended to this wavefront's queue in LDS, and the integral code runs only on
+    // full groups of 64 queued kets (all lanes busy), plus one partial group at the end.
+    // The queue is private to the wavefront, so ordering its LDS traffic needs no
+    // work-group barrier.
+    int2 *q = kqueue[threadIdx.x >> 6];
+    int qn = 0;                             // wave-uniform
     constexpr int step = KROWS ? 256 : 64;  // KROWS: the four wavefronts interleave rounds
     for (int j0 = KROWS ? 64 * wave : 0;; j0 += step) {
         if (j0 < b.jend) {
             const int j = j0 + lane;
             KetTest t;
             const bool ok = j < b.jend && test_ket(a, b, j, t);
-            const unsigned long long m1 = __ballot(ok && t.skip_lists), m0 = __ballot(ok && !t.skip_lists);
+            const unsigned long long m = __ballot(ok);
             if (ok) {
-                const unsigned long long m = t.skip_lists ? m1 : m0;
-                const int pos = (t.skip_lists ? qn1 : qn0) +
-                                (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                q[t.skip_lists ? 1 : 0][pos] =
-                    make_int2((j << 5) | (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) |
-                                  (t.halvings << 3),
-                              (t.fr << 16) | t.fs);
+                const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                q[pos] = make_int2((j << 5) | (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) |
+                                       (t.halvings << 3),
+                                   (t.fr << 16) | t.fs);
             }
-            qn0 += __popcll(m0);
-            qn1 += __popcll(m1);
+            qn += __popcll(m);
             wave_lds_sync();
         }
         const bool drain = j0 + step >= b.jend;  // nothing left to test
-        for (;;) {
-            // a full group from either queue, or (draining) what is left
-            const int L = qn1 >= 64 ? 1 : qn0 >= 64 ? 0 : !drain ? -1 : qn1 > 0 ? 1 : qn0 > 0 ? 0 : -1;
-            if (L < 0) break;
-            int &qn = L ? qn1 : qn0;
+        if (qn >= 64 || (drain && qn > 0)) {
+            // take the last min(qn, 64) entries; later rounds append behind the rest
             const int take = qn < 64 ? qn : 64, base = qn - take;
-            const int2 e = lane < take ? q[L][base + lane] : make_int2(0, 0);
+            const int2 e = lane < take ? q[base + lane] : make_int2(0, 0);
             qn = base;
             wave_lds_sync();
-            const PairView bv = make_view(a.pr, a.pm, b.sb, L == 1);  // wave-uniform
-            if (lane < take) quartet(e.x >> 5, e.x & 31, e.y >> 16, e.y & 0xffff, bv);
+            if (lane < take) quartet(e.x >> 5, e.x & 31, e.y >> 16, e.y & 0xffff,
+                                     make_view(a.pr, a.pm, b.sb, e.x & 4));
         }
-        if (drain) break;
+        if (drain && qn == 0) break;
     }
 #else
     for (int j0 = KROWS ? 64 * wave : 0; j0 < b.jend; j0 += KROWS ? 256 : 64) {

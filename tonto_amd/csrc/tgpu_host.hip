@@ -887,6 +887,9 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
                 a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
                 a.same_bin = X == Y;
                 a.n_ket_max = c->bin_max_seg[Y];
+                // the LDS-row form is for single large molecules (0 keeps batches of many,
+                // mostly small molecules at four bra pairs per work-group)
+                a.n_ket_avg = c->n_mol == 1 ? c->bin_start[Y + 1] - c->bin_start[Y] : 0;
                 // wavefront form: one lane screens each ket of a chunk; chunks of 64 unless
                 // that leaves fewer than ~128k work-groups, then 32 or 16
                 a.ket_chunk = 64;
