@@ -58,6 +58,30 @@ def test_water_all_classes(torch_cuda, oracle_mod, basis):
     _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 1))
 
 
+@pytest.mark.parametrize("budget", [None, 20480])
+def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
+    """The lane-per-quartet kernels' LDS K-row form on (H2O)_8 / 6-31G(d) (152 functions),
+    made to run for every ket segment length; with a 20 KiB LDS cap the (pp|ss), (pp|ps),
+    (ds|ss), (ds|ps) launches keep only 4 of their 6-7 K rows in LDS and update the rest
+    with global atomics."""
+    import os
+    from tonto_amd.geometry import water_cluster
+    g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))
+    mol = water_cluster(8, "6-31G(d)", seed=int(g["seed"]))
+    old = {k: os.environ.get(k) for k in ("TGPU_KROW_BUDGET", "TGPU_KROW_MIN_KETS")}
+    os.environ["TGPU_KROW_MIN_KETS"] = "0"
+    if budget:
+        os.environ["TGPU_KROW_BUDGET"] = str(budget)
+    try:
+        _check_jk(oracle_mod, mol, g["P"], cut=1e-12)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
 def test_f_shells_water_cc_pvtz(torch_cuda, oracle_mod):
     """cc-pVTZ water: f on O, d on H; (ff|ff) and every other f class pair, including
     the classes whose transfer buffers live in the global workspace."""

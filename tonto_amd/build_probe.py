@@ -1,10 +1,12 @@
 """Per-class timing of one J/K build of a seeded water cluster (a profiling aid).
 
-    python tonto_amd/build_probe.py [basis] [n_waters] [schwarz_cutoff]
+    python tonto_amd/build_probe.py [basis] [n_waters] [schwarz_cutoff] [scf_iterations]
 
-Sets up the GPU SCF context, forms the promolecule guess, runs two J/K builds at that
-density with per-class HIP-event timing, and prints quartet counts, times and the
-algorithmic TFLOP/s (tonto_amd/flops.py) of every class pair that has quartets.
+Sets up the GPU SCF context, forms the promolecule guess, optionally runs a few FP64 SCF
+iterations from it (the density then resembles the converged one, which screens far
+fewer quartets away than the block-diagonal guess), runs two J/K builds at that density
+with per-class HIP-event timing, and prints quartet counts, times and the algorithmic
+TFLOP/s (tonto_amd/flops.py) of every class pair that has quartets.
 """
 import os
 import sys
@@ -21,6 +23,7 @@ def main(argv):
     basis = argv[0] if len(argv) > 0 else "cc-pVTZ"
     nw = int(argv[1]) if len(argv) > 1 else 64
     cut = float(argv[2]) if len(argv) > 2 else 1e-11
+    n_it = int(argv[3]) if len(argv) > 3 else 0
     mol = water_cluster(nw, basis)
     print(f"(H2O){nw} {basis}: {mol.n_bf} basis functions, {len(mol.shells)} shells", flush=True)
     t = time.perf_counter()
@@ -31,6 +34,14 @@ def main(argv):
     P0 = s.guess()
     torch.cuda.synchronize()
     print(f"guess {time.perf_counter() - t:.2f} s", flush=True)
+    if n_it:
+        from tonto_amd.scf import run_rhf
+        t = time.perf_counter()
+        r = run_rhf(s.h, s.S, mol.n_electrons, mol.nuclear_repulsion(), s.jk, P0=P0,
+                    max_iterations=n_it, log=lambda m: print(m, flush=True))
+        torch.cuda.synchronize()
+        P0 = r.P
+        print(f"{n_it} SCF iterations {time.perf_counter() - t:.1f} s, E = {r.energy:.8f}", flush=True)
     s.engine.set_profiling(True)
     for it in range(2):
         torch.cuda.synchronize()

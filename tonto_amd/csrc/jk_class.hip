@@ -22,30 +22,36 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
 {
 #if !TGPU_HEAVY
     if (variant == 0) {
-        // the LDS-row form for (ss|, (ps| bra pairs, and (pp|, (ds| against (ss| and (ps|
-        // kets (2-7 K rows, kernels well below the register limit), when the rows fit a
-        // 96 KiB work-group next to the Rys table and the ket queues,
+        // The LDS-row form for single molecules with long ket segments (>= 4 rounds of 64 by
+        // default, so that the four wavefronts of one bra pair have work), in the classes whose
+        // lane-per-quartet kernel stays well below the register limit: as many of the bra
+        // pair's K rows as fit the work-group's LDS budget next to the Rys table and the
+        // ket queue, and at least half of them.
         constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
-#if (TGPU_LP + TGPU_LQ <= 1) || (TGPU_LP + TGPU_LQ == 2 && TGPU_LR + TGPU_LS <= 1)
+#if (TGPU_LP + TGPU_LQ <= 1) || (TGPU_LP + TGPU_LQ == 2 && TGPU_LR + TGPU_LS <= 1) || \
+    (TGPU_LP + TGPU_LQ == 3 && TGPU_LR + TGPU_LS == 0)
         constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
         constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 128 * 8 : 0);
-        constexpr size_t kBudget = 96 * 1024;
+        constexpr size_t kMaxLds = 155 * 1024;
         static bool attr_set = false;
         if (!attr_set) {
             // dynamic LDS above the 64 KiB default has to be allowed per kernel
             (void)hipFuncSetAttribute((const void *)jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kBudget - kStatic));
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kMaxLds - kStatic));
             (void)hipFuncSetAttribute((const void *)jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kBudget - kStatic));
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kMaxLds - kStatic));
             attr_set = true;
         }
-        const size_t rows = (size_t)(NP + NQ) * a.max_nbf * sizeof(double);
-        // and when the molecules' ket segments are on average long enough (>= 4 rounds of
-        // 64) to keep the four wavefronts of one bra pair busy: batches of small molecules
-        // keep four bra pairs per work-group
-        if (a.max_nbf > 0 && rows + kStatic <= kBudget && a.n_ket_avg >= 256) {
-            if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(a);
-            else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(a);
+        const size_t budget = a.krow_budget > 0 && (size_t)a.krow_budget < kMaxLds ? (size_t)a.krow_budget : kMaxLds;
+        const size_t row_bytes = (size_t)a.max_nbf * sizeof(double);
+        const int fit = a.max_nbf > 0 && budget > kStatic ? (int)((budget - kStatic) / row_bytes) : 0;
+        const int nkr = fit < NP + NQ ? fit : NP + NQ;
+        if (a.n_ket_avg >= a.krow_min_kets && nkr > 0 && 2 * nkr >= NP + NQ) {
+            JKArgs b = a;
+            b.krow_rows = nkr;
+            const size_t rows = (size_t)nkr * row_bytes;
+            if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(b);
+            else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(b);
             return;
         }
 #else

@@ -242,6 +242,9 @@ struct JKArgs {
     double *ws;    // wavefront-form workspace of ws_doubles doubles: a class uses
     long long ws_doubles;  // min(8192, ws_doubles / WaveCfg::WS) slots
     int max_nbf;   // largest molecule (basis functions): the LDS rows of the KROWS form
+    int krow_rows;   // KROWS form: the first krow_rows K rows of the bra pair live in LDS
+    int krow_budget; // cap on a work-group's LDS in the KROWS form (bytes; 0: default)
+    int krow_min_kets;  // shortest ket segment (n_ket_avg) for the KROWS form
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -356,10 +359,11 @@ __device__ __forceinline__ void wave_lds_sync()
 // The J/K kernel for one light class pair (LP LQ | LR LS): one lane per quartet
 // ----------------------------------------------------------------------------------
 // KROWS: one bra pair per work-group, its four wavefronts taking interleaved rounds of the
-// ket prefix; the K updates of the bra pair (all in its NP + NQ rows) accumulate in LDS
-// (dynamic shared memory, (NP + NQ) x n_bf doubles) and are flushed once, row by row,
-// with coalesced atomics.  Otherwise four bra pairs per work-group, one per wavefront,
-// and every K update is a global atomic.
+// ket prefix; the K updates of the bra pair all fall in its NP + NQ rows, and the first
+// krow_rows of them (all, when n_bf allows) accumulate in LDS (dynamic shared memory,
+// krow_rows x n_bf doubles) and are flushed once, row by row, with coalesced atomics; the
+// other rows are updated with global atomics.  Otherwise four bra pairs per work-group,
+// one per wavefront, and every K update is a global atomic.
 template <int LP, int LQ, int LR, int LS, bool MIXED, bool KROWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves(LP, LQ, LR, LS), 8))) void jk_kernel(JKArgs a)
 {
@@ -388,13 +392,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     BraInfo b;
     load_bra(a, i, b);
     const int ld = b.ld, fp = b.fp, fq = b.fq;
+    const int nkr = KROWS ? a.krow_rows : 0;  // K rows accumulated in LDS
     if (KROWS) {
-        for (int t = threadIdx.x; t < (NP + NQ) * ld; t += blockDim.x) krow[t] = 0.0;
+        for (int t = threadIdx.x; t < nkr * ld; t += blockDim.x) krow[t] = 0.0;
         __syncthreads();
     }
     // K(row, col) += v: row = bra function fp + p (p < NP) or fq + q (NP + q)
     auto kadd = [&](const int row, const int col, const double v) {
-        if (KROWS) {
+        if (KROWS && row < nkr) {
             __hip_atomic_fetch_add(&krow[row * ld + col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     if (KROWS) {
         // every wavefront of the work-group is here (one bra pair, no early return)
         __syncthreads();
-        for (int t = threadIdx.x; t < (NP + NQ) * ld; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nkr * ld; t += blockDim.x) {
             const double v = krow[t];
             const int row = t / ld, col = t - row * ld;
             if (v != 0.0) unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);

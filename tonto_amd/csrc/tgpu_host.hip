@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -307,6 +308,10 @@ struct tgpu_ctx {
     // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
     DBuf<double> d_ws;
     long long ws_doubles = 0;
+    // LDS cap of the lane-per-quartet kernels' K-row form (bytes, 0: the kernel's own);
+    // TGPU_KROW_BUDGET sets it (tests use it to exercise partial rows on small systems)
+    int krow_budget = 0;
+    int krow_min_kets = 256;  // TGPU_KROW_MIN_KETS
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
     bool timed = false;
     bool profiling = false;
@@ -524,6 +529,8 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
         return TGPU_E_HIP;
     }
     c->n_mol = n_mol;
+    if (const char *kb = std::getenv("TGPU_KROW_BUDGET")) c->krow_budget = std::atoi(kb);
+    if (const char *km = std::getenv("TGPU_KROW_MIN_KETS")) c->krow_min_kets = std::atoi(km);
     c->atom_off.assign(n_mol + 1, 0);
     c->shell_off.assign(n_mol + 1, 0);
     c->pair_off.assign(n_mol + 1, 0);
@@ -868,6 +875,9 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.ws = c->d_ws.p;
     a.ws_doubles = c->ws_doubles;
     a.max_nbf = *std::max_element(c->nbf.begin(), c->nbf.end());
+    a.krow_rows = 0;
+    a.krow_budget = c->krow_budget;
+    a.krow_min_kets = c->krow_min_kets;
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];
         const bool prof = timing && c->profiling;
