@@ -192,6 +192,21 @@ def test_gpu_scf_reaches_g09(torch_cuda):
     assert r.converged and abs(r.energy - (-76.0232731206)) < 1e-9
 
 
+def test_mixed_incremental_scf_energy(torch_cuda):
+    """FP32 small-bound quartets plus incremental (delta-density) builds converge to the
+    FP64 energy within 1e-8 Eh on (H2O)8 / 6-31G(d,p)."""
+    from tonto_amd.geometry import water_cluster
+    from tonto_amd.driver import GPUSCF
+    scf = GPUSCF(water_cluster(8, "6-31G(d,p)", seed=1), schwarz=1e-11, fp32_threshold=1e-6)
+    r64 = scf.run(precision="fp64", convergence=1e-9, diis_tol=1e-6)
+    rmx = scf.run(precision="mixed", mixed_until=1e-5, incremental=True, rebuild_every=8,
+                  convergence=1e-9, diis_tol=1e-6)
+    print(r64.energy, rmx.energy - r64.energy, rmx.n_builds)
+    assert r64.converged and rmx.converged
+    assert abs(rmx.energy - r64.energy) <= 1e-8
+    assert rmx.n_builds["delta"] + rmx.n_builds["delta_mixed"] > 0
+
+
 def _uneven_batch(n_mol=6, basis="6-31G(d,p)", seed=7):
     from tonto_amd.geometry import water_cluster
     rng = np.random.default_rng(seed)

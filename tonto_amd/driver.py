@@ -36,6 +36,7 @@ class GPUSCF:
         self._J = torch.empty((n, n), dtype=torch.float64, device=self.device)
         self._K = torch.empty((n, n), dtype=torch.float64, device=self.device)
         self.graph = False
+        self.delta_scale = 1e-2
 
     def autotune(self, P, mixed=False):
         """Pick the faster kernel form per class pair on this molecule and density."""
@@ -102,11 +103,12 @@ class GPUSCF:
         return torch.tensor(P, device=self.device)
 
     def jk_incremental(self, P, mode):
-        """Delta builds are screened with a 1000x tighter Schwarz cutoff than full builds:
-        the contributions a full build drops are bounded by the cutoff times |P|, a delta
-        build's by the cutoff times |dP|, and those errors would otherwise accumulate."""
+        """Delta builds are screened with a tighter Schwarz cutoff than full builds
+        (delta_scale times it, default 1e-2): the contributions a full build drops are
+        bounded by the cutoff times |P|, a delta build's by the cutoff times |dP|, and those
+        errors would otherwise accumulate."""
         if mode.startswith("delta"):
-            self.engine.set_cutoffs_scaled(1e-3)
+            self.engine.set_cutoffs_scaled(self.delta_scale)
             try:
                 return self.jk(P, mode)
             finally:

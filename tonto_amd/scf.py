@@ -86,7 +86,8 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
             max_iterations=100, min_iterations=1, diis_keep=8, diis_start=3,
             diis_start_error=None,
             level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3,
-            mixed_until=None, incremental=False, rebuild_every=8, log=None):
+            mixed_until=None, incremental=False, rebuild_every=8, incremental_from=1e-2,
+            log=None):
     """Run RHF to convergence.
 
     jk(P, mode) returns (J, K); mode is "full" or "delta" (a build from a density
@@ -95,7 +96,9 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
       |dE| < mixed_until, then a full FP64 build is forced and FP64 is kept to the end.
     incremental: update the 2e Fock part with builds on P - P_prev; every rebuild_every-th
       iteration, at the precision switch and once converged a full build is done, and
-      convergence is only accepted on a full FP64 build.
+      convergence is only accepted on a full FP64 build.  While the DIIS error is still at
+      or above incremental_from, P - P_prev is as large as P and a difference build (run
+      at a tighter cutoff) would cost more than a full one, so full builds are used.
     """
     dt = h.dtype
     n_occ = n_electrons // 2
@@ -110,6 +113,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     history = []
     diis_F, diis_E = [], []
     n_builds = {"full": 0, "delta": 0, "full_mixed": 0, "delta_mixed": 0}
+    build_s = dict.fromkeys(n_builds, 0.0)  # wall time of the builds by kind
     n_builds["full_mixed" if low else "full"] += 1
     P_build = P
     last_full_fp64 = not low
@@ -156,14 +160,18 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         P_new = 2.0 * Co @ Co.T
         if damp and (it < damp_iters or not diis_on):
             P_new = damp * P + (1.0 - damp) * P_new
-        full = (not incremental) or force_full or it % rebuild_every == 0
+        full = (not incremental) or force_full or it % rebuild_every == 0 or err_max >= incremental_from
         mode = ("full" if full else "delta") + ("_mixed" if low else "")
+        tb = time.perf_counter()
         if full:
             J, K = jk(P_new, mode)
             G = J - 0.5 * K
         else:
             dJ, dK = jk(P_new - P_build, mode)
             G = G + dJ - 0.5 * dK
+        if G.is_cuda:
+            torch.cuda.synchronize(G.device)
+        build_s[mode] += time.perf_counter() - tb
         n_builds[mode] += 1
         force_full = False
         last_full_fp64 = full and not low
@@ -199,4 +207,4 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
             break
     wall = time.perf_counter() - t0
     return SCFResult(energy=E, converged=converged, iterations=it, P=P, F=F, C=C, eps=eps,
-                     history=history, wall=wall, n_builds=n_builds)
+                     history=history, wall=wall, n_builds=n_builds, build_seconds=build_s)
