@@ -1,12 +1,13 @@
 """Per-class timing of one J/K build of a seeded water cluster (a profiling aid).
 
-    python tonto_amd/build_probe.py [basis] [n_waters] [schwarz_cutoff] [scf_iterations]
+    python tonto_amd/build_probe.py [basis] [n_waters] [schwarz_cutoff] [scf_iterations] [mode]
 
 Sets up the GPU SCF context, forms the promolecule guess, optionally runs a few FP64 SCF
 iterations from it (the density then resembles the converged one, which screens far
 fewer quartets away than the block-diagonal guess), runs two J/K builds at that density
-with per-class HIP-event timing, and prints quartet counts, times and the algorithmic
-TFLOP/s (tonto_amd/flops.py) of every class pair that has quartets.
+with per-class HIP-event timing (mode "full", the default, or "full_mixed"), and prints
+quartet counts (FP32 ones too), times and the algorithmic TFLOP/s (tonto_amd/flops.py) of
+every class pair that has quartets.
 """
 import os
 import sys
@@ -24,6 +25,7 @@ def main(argv):
     nw = int(argv[1]) if len(argv) > 1 else 64
     cut = float(argv[2]) if len(argv) > 2 else 1e-11
     n_it = int(argv[3]) if len(argv) > 3 else 0
+    mode = argv[4] if len(argv) > 4 else "full"
     mol = water_cluster(nw, basis)
     print(f"(H2O){nw} {basis}: {mol.n_bf} basis functions, {len(mol.shells)} shells", flush=True)
     t = time.perf_counter()
@@ -46,7 +48,7 @@ def main(argv):
     for it in range(2):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        s.jk(P0)
+        s.jk(P0, mode)
         torch.cuda.synchronize()
         print(f"build {it} {1e3 * (time.perf_counter() - t):.1f} ms", flush=True)
     st = s.engine.stats()
@@ -58,8 +60,8 @@ def main(argv):
     print(f"build flops {tot:.3e} -> {tot / (st['kernel_ms'] * 1e-3) / 1e12:.2f} TF")
     for k, cp in enumerate(class_pairs()):
         if cc[k][0]:
-            print("(%d%d|%d%d) q=%11d prim=%12d %9.3f ms %5.2f TF form=%s"
-                  % (cp + (cc[k][0], cc[k][1], ct[k], per[k] / max(ct[k], 1e-9) / 1e9,
+            print("(%d%d|%d%d) q=%11d fp32=%11d prim=%12d %9.3f ms %5.2f TF form=%s"
+                  % (cp + (cc[k][0], cc[k][2], cc[k][1], ct[k], per[k] / max(ct[k], 1e-9) / 1e9,
                            "wave" if forms[k] else "lane")), flush=True)
 
 

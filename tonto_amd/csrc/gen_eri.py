@@ -103,13 +103,14 @@ def gen_class(LP, LQ, LR, LS, R):
     # is uniform: ldc() then makes them scalar loads)
     em("for (int iq = 0; iq < ket.n; ++iq) {", 2)
     em("const int iqq = ket.off + iq;", 3)
-    em("const double zc = pm.zeta[iqq], izc = pm.zinv[iqq], Kc = pm.K[iqq];", 3)
-    em("const double Qx = pm.Px[iqq], Qy = pm.Py[iqq], Qz = pm.Pz[iqq];", 3)
+    em("const PrimRec *__restrict__ pk = pm.rec + iqq;", 3)
+    em("const double zc = pk->zeta, izc = pk->zinv, Kc = pk->K;", 3)
+    em("const double Qx = pk->Px, Qy = pk->Py, Qz = pk->Pz;", 3)
     em(f"const {R} QCx = ({R})(Qx - ket.Ax), QCy = ({R})(Qy - ket.Ay), QCz = ({R})(Qz - ket.Az);", 3)
     em("for (int ip = 0; ip < bra.n; ++ip) {", 3)
     em("const int ipp = bra.off + ip;", 4)
-    em("const double za = ldc(pm.zeta + ipp), iza = ldc(pm.zinv + ipp), Ka = ldc(pm.K + ipp);", 4)
-    em("const double Px = ldc(pm.Px + ipp), Py = ldc(pm.Py + ipp), Pz = ldc(pm.Pz + ipp);", 4)
+    em("const PrimRec pb = ldc(pm.rec + ipp);", 4)
+    em("const double za = pb.zeta, iza = pb.zinv, Ka = pb.K, Px = pb.Px, Py = pb.Py, Pz = pb.Pz;", 4)
     em(f"const {R} PAx = ({R})(Px - bra.Ax), PAy = ({R})(Py - bra.Ay), PAz = ({R})(Pz - bra.Az);", 4)
     em("const double izs = 1.0 / (za + zc), rho = za * zc * izs;", 4)
     em("const double QPx = Qx - Px, QPy = Qy - Py, QPz = Qz - Pz;", 4)

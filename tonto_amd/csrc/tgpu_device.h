@@ -14,8 +14,15 @@
 // ----------------------------------------------------------------------------------
 // Device helpers
 // ----------------------------------------------------------------------------------
+// One primitive pair of a shell pair: exponent sum, its inverse, prefactor and product
+// centre, packed in one 64-byte record so that a lane gathering its ket's primitive pair
+// touches one cache line (one scalar load sequence for a wavefront-uniform bra).
+struct __attribute__((aligned(64))) PrimRec {
+    double zeta, zinv, K, Px, Py, Pz, pad0, pad1;
+};
+
 struct DevPrims {
-    const double *zeta, *zinv, *K, *Px, *Py, *Pz;
+    const PrimRec *rec;
 };
 
 // Load from memory that no kernel of the build writes (pair and primitive tables) through
@@ -534,11 +541,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
                 }
         }
     };
-    // Compaction: each round the 64 lanes test 64 kets of the prefix; the kets that pass
-    // are appended to this wavefront's queue in LDS, and the integral code runs only on
-    // full groups of 64 queued kets (all lanes busy), plus one partial group at the end.
-    // The queue is private to the wavefront, so ordering its LDS traffic needs no
-    // work-group barrier.
 #if TGPU_COMPACT
     // Compaction: each round the 64 lanes test 64 kets of the prefix; the kets that pass
     // are appended to this wavefront's queue in LDS, and the integral code runs only on
@@ -723,13 +725,13 @@ __device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const
     const int ax = lane / N, rr = lane - (lane / N) * N;
     for (int ip = 0; ip < bra.n; ++ip) {
         const int ipp = bra.off + ip;
-        const double za = ldc(pm.zeta + ipp), Ka = ldc(pm.K + ipp);
-        const double Px = ldc(pm.Px + ipp), Py = ldc(pm.Py + ipp), Pz = ldc(pm.Pz + ipp);
+        const PrimRec pb = ldc(pm.rec + ipp);
+        const double za = pb.zeta, Ka = pb.K, Px = pb.Px, Py = pb.Py, Pz = pb.Pz;
         for (int iq = 0; iq < ket.n; ++iq) {
             if (lane < 3 * N) {
                 const int iqq = ket.off + iq;
-                const double zc = ldc(pm.zeta + iqq), Kc = ldc(pm.K + iqq);
-                const double Qx = ldc(pm.Px + iqq), Qy = ldc(pm.Py + iqq), Qz = ldc(pm.Pz + iqq);
+                const PrimRec pk = ldc(pm.rec + iqq);
+                const double zc = pk.zeta, Kc = pk.K, Qx = pk.Px, Qy = pk.Py, Qz = pk.Pz;
                 const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
                 const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
                 const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);

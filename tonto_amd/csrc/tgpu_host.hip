@@ -302,7 +302,8 @@ struct tgpu_ctx {
     DBuf<int> d_rec0, d_rec1, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
         d_atom_off;
     DBuf<long long> d_pair_off, d_mat_off;
-    DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol, d_pz, d_pzi, d_pK, d_pPx, d_pPy, d_pPz;
+    DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol;
+    DBuf<PrimRec> d_prim;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
     DBuf<unsigned long long> d_counters;
     // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
@@ -456,12 +457,12 @@ static int upload_pairs(tgpu_ctx *c)
     int rc = 0;
     std::vector<int> r0, r1;
     pack_records(c, r0, r1);
+    std::vector<PrimRec> prim(c->pz.size());
+    for (size_t k = 0; k < prim.size(); ++k)
+        prim[k] = PrimRec{c->pz[k], c->pzi[k], c->pK[k], c->pPx[k], c->pPy[k], c->pPz[k], 0.0, 0.0};
     if ((rc = dupload(c, c->d_rec0, r0)) || (rc = dupload(c, c->d_rec1, r1)) ||
         (rc = dupload(c, c->d_A, c->A)) || (rc = dupload(c, c->d_AB, c->AB)) ||
-        (rc = dupload(c, c->d_pz, c->pz)) || (rc = dupload(c, c->d_pzi, c->pzi)) ||
-        (rc = dupload(c, c->d_pK, c->pK)) ||
-        (rc = dupload(c, c->d_pPx, c->pPx)) || (rc = dupload(c, c->d_pPy, c->pPy)) ||
-        (rc = dupload(c, c->d_pPz, c->pPz)) || (rc = dupload(c, c->d_seg_start, c->seg_start)) ||
+        (rc = dupload(c, c->d_prim, prim)) || (rc = dupload(c, c->d_seg_start, c->seg_start)) ||
         (rc = dupload(c, c->d_seg_count, c->seg_count)))
         return rc;
     if (c->maxI.size() == c->tonto.size()) rc = dupload(c, c->d_maxI, c->maxI);
@@ -482,7 +483,7 @@ static DevPairs dev_pairs(const tgpu_ctx *c)
 static DevPrims dev_prims(const tgpu_ctx *c)
 {
     DevPrims d;
-    d.zeta = c->d_pz.p; d.zinv = c->d_pzi.p; d.K = c->d_pK.p; d.Px = c->d_pPx.p; d.Py = c->d_pPy.p; d.Pz = c->d_pPz.p;
+    d.rec = c->d_prim.p;
     return d;
 }
 
@@ -676,8 +677,8 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_rec0); dfree(c->d_rec1); dfree(c->d_sh_l); dfree(c->d_sh_bf);
     dfree(c->d_seg_start); dfree(c->d_seg_count); dfree(c->d_nbf); dfree(c->d_shell_off);
     dfree(c->d_atom_off); dfree(c->d_pair_off); dfree(c->d_mat_off);
-    dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_pz); dfree(c->d_pzi);
-    dfree(c->d_pK); dfree(c->d_pPx); dfree(c->d_pPy); dfree(c->d_pPz); dfree(c->d_X); dfree(c->d_Y);
+    dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_prim);
+    dfree(c->d_X); dfree(c->d_Y);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
     dfree(c->d_counters);
     dfree(c->d_ws);

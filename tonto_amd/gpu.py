@@ -142,10 +142,12 @@ class JKEngine:
         except Exception:
             pass
 
-    def set_cutoffs_scaled(self, scale):
-        """Schwarz and density cutoffs times `scale` (the pair cutoff is kept)."""
+    def set_cutoffs_scaled(self, scale, density_scale=None):
+        """Schwarz cutoff times `scale`, the J and K density cutoffs times `density_scale`
+        (default: `scale`); the pair cutoff is kept."""
         s, j, k, p = self.cutoffs
-        self._check(lib().tgpu_set_cutoffs(self._ctx, s * scale, j * scale, k * scale, p))
+        d = scale if density_scale is None else density_scale
+        self._check(lib().tgpu_set_cutoffs(self._ctx, s * scale, j * d, k * d, p))
 
     def set_stream(self, stream_ptr):
         self._check(lib().tgpu_set_stream(self._ctx, ctypes.c_void_p(stream_ptr)))

@@ -87,16 +87,27 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
             diis_start_error=None,
             level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3,
             mixed_until=None, incremental=False, rebuild_every=8, incremental_from=1e-2,
+            loose_scale=1.0, loose_until=None, rebuild_after_loose=True, diis_reset_on_switch=False,
             log=None):
     """Run RHF to convergence.
 
     jk(P, mode) returns (J, K); mode is "full" or "delta" (a build from a density
-    difference), with the suffix "_mixed" while low precision is allowed.
+    difference), with the suffix "_mixed" while low precision is allowed; jk(P, mode, scale)
+    is called for builds whose screening cutoffs are to be scale times the usual ones.
     mixed_until: None for FP64 builds throughout; otherwise builds are "_mixed" until
       |dE| < mixed_until, then a full FP64 build is forced and FP64 is kept to the end.
-    incremental: update the 2e Fock part with builds on P - P_prev; every rebuild_every-th
-      iteration, at the precision switch and once converged a full build is done, and
-      convergence is only accepted on a full FP64 build.  While the DIIS error is still at
+    loose_scale: cutoff scale of the "_mixed" builds made while the DIIS error is still
+      at or above loose_until (default incremental_from; the first build included).  They
+      are full builds, their screening error (~1e-5 in F at a 1e-8 cutoff) stays below the
+      SCF error then, and the next full build at the usual cutoffs (at the latest the FP64
+      one at the precision switch) removes it.
+    rebuild_after_loose: make the first build after the loose ones a full build at the
+      usual cutoffs; otherwise delta builds go on from the loose G and the FP64 build at
+      the precision switch is the first one without the loose error.
+    diis_reset_on_switch: drop the DIIS history at the precision switch.
+    incremental: update the 2e Fock part with builds on P - P_prev; after rebuild_every
+      builds since the last full one, at the precision switch and once converged a full
+      build is done, and convergence is only accepted on a full FP64 build.  While the DIIS error is still at
       or above incremental_from, P - P_prev is as large as P and a difference build (run
       at a tighter cutoff) would cost more than a full one, so full builds are used.
     """
@@ -105,17 +116,30 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     X = _sym_orth(S)
     P = core_guess(h, X, n_occ) if P0 is None else P0.clone()
     low = mixed_until is not None
+    loose_until = incremental_from if loose_until is None else loose_until
+    n_builds = {"full": 0, "delta": 0, "full_mixed": 0, "delta_mixed": 0}
+    build_s = dict.fromkeys(n_builds, 0.0)  # wall time of the builds by kind
+
+    def build(D, mode, loose=False):
+        scale = loose_scale if loose and mode == "full_mixed" else 1.0
+        tb = time.perf_counter()
+        J, K = jk(D, mode) if scale == 1.0 else jk(D, mode, scale)
+        if J.is_cuda:
+            torch.cuda.synchronize(J.device)
+        build_s[mode] += time.perf_counter() - tb
+        n_builds[mode] += 1
+        return J, K
+
     t0 = time.perf_counter()
-    J, K = jk(P, "full_mixed" if low else "full")
+    J, K = build(P, "full_mixed" if low else "full", loose=True)
     G = J - 0.5 * K
     F = h + G
     E = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
     history = []
     diis_F, diis_E = [], []
-    n_builds = {"full": 0, "delta": 0, "full_mixed": 0, "delta_mixed": 0}
-    build_s = dict.fromkeys(n_builds, 0.0)  # wall time of the builds by kind
-    n_builds["full_mixed" if low else "full"] += 1
     P_build = P
+    since_full = 0  # builds since the last full one
+    loose_G = low and loose_scale != 1.0  # G holds a loose-cutoff build
     last_full_fp64 = not low
     force_full = False
     converged = False
@@ -123,6 +147,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     it = 0
     while True:
         it += 1
+        ti = time.perf_counter()
         err = X.T @ (F @ P @ S - S @ P @ F) @ X
         err_max = err.abs().max().item()
         diis_F.append(F)
@@ -151,7 +176,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
                 F_use = F
         Fp = X.T @ F_use @ X
         if level_shift and it < level_shift_iters:
-            Xi = torch.linalg.pinv(X)
+            Xi = S @ X  # X = S^-1/2, so X^-1 = S^1/2 = S X
             Cocc_p = Xi @ (P / 2.0) @ Xi.T
             Fp = Fp + level_shift * (torch.eye(Fp.shape[0], dtype=dt, device=h.device) - Cocc_p)
         eps, Cp = torch.linalg.eigh(Fp)
@@ -160,19 +185,20 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         P_new = 2.0 * Co @ Co.T
         if damp and (it < damp_iters or not diis_on):
             P_new = damp * P + (1.0 - damp) * P_new
-        full = (not incremental) or force_full or it % rebuild_every == 0 or err_max >= incremental_from
+        # a G from a loose-cutoff build is replaced by a full build before any delta
+        full = ((not incremental) or force_full or since_full + 1 >= rebuild_every or
+                err_max >= incremental_from or (loose_G and rebuild_after_loose))
         mode = ("full" if full else "delta") + ("_mixed" if low else "")
         tb = time.perf_counter()
         if full:
-            J, K = jk(P_new, mode)
+            loose_G = low and loose_scale != 1.0 and err_max >= loose_until
+            J, K = build(P_new, mode, loose=loose_G)
             G = J - 0.5 * K
         else:
-            dJ, dK = jk(P_new - P_build, mode)
+            dJ, dK = build(P_new - P_build, mode)
             G = G + dJ - 0.5 * dK
-        if G.is_cuda:
-            torch.cuda.synchronize(G.device)
-        build_s[mode] += time.perf_counter() - tb
-        n_builds[mode] += 1
+        tb = time.perf_counter() - tb
+        since_full = 0 if full else since_full + 1
         force_full = False
         last_full_fp64 = full and not low
         P_build = P_new
@@ -181,19 +207,23 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         E_new = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
         dE = E_new - E
         E = E_new
-        history.append(dict(iteration=it, energy=E, dE=dE, diis_error=err_max, mode=mode))
+        history.append(dict(iteration=it, energy=E, dE=dE, diis_error=err_max, mode=mode,
+                            build_seconds=tb, seconds=time.perf_counter() - ti))
         if log:
-            log(f"iter {it:3d} E={E:.12f} dE={dE:+.3e} err={err_max:.3e} {mode}")
+            log(f"iter {it:3d} E={E:.12f} dE={dE:+.3e} err={err_max:.3e} {mode} "
+                f"build {tb:.3f}s iteration {history[-1]['seconds']:.3f}s")
         done = abs(dE) < convergence and err_max < diis_tol
         if low:
             if abs(dE) < mixed_until or done:
                 low = False
                 force_full = True
+                if diis_reset_on_switch:
+                    diis_F.clear()
+                    diis_E.clear()
             done = False
         elif done and not last_full_fp64:
             # win back the incremental error: a full FP64 build on the current density
-            J, K = jk(P, "full")
-            n_builds["full"] += 1
+            J, K = build(P, "full")
             G = J - 0.5 * K
             F = h + G
             E_full = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
