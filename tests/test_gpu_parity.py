@@ -193,8 +193,9 @@ def test_gpu_scf_reaches_g09(torch_cuda):
 
 
 def test_mixed_incremental_scf_energy(torch_cuda):
-    """FP32 small-bound quartets plus incremental (delta-density) builds converge to the
-    FP64 energy within 1e-8 Eh on (H2O)8 / 6-31G(d,p)."""
+    """FP32 small-bound quartets, loose-cutoff first builds and incremental
+    (delta-density) builds converge to the FP64 energy within 1e-8 Eh on
+    (H2O)8 / 6-31G(d,p)."""
     from tonto_amd.geometry import water_cluster
     from tonto_amd.driver import GPUSCF
     scf = GPUSCF(water_cluster(8, "6-31G(d,p)", seed=1), schwarz=1e-11, fp32_threshold=1e-6)

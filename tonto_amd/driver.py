@@ -36,9 +36,12 @@ class GPUSCF:
         self._J = torch.empty((n, n), dtype=torch.float64, device=self.device)
         self._K = torch.empty((n, n), dtype=torch.float64, device=self.device)
         self.graph = False
-        self.delta_scale = 1e-3
-        self.mixed_delta_scale = 10.0
-        self.mixed_delta_density_scale = 1e-3
+        self.delta_cutoff = 1e-14
+        self.delta_density_cutoff = 1e-12
+        self.mixed_delta_cutoff = 1e-10
+        self.mixed_delta_density_cutoff = 1e-12
+        # Schwarz cutoff of the mixed run's first full builds (see run)
+        self.loose_cutoff = 1e-8
 
     def autotune(self, P, mixed=False):
         """Pick the faster kernel form per class pair on this molecule and density."""
@@ -118,16 +121,21 @@ class GPUSCF:
         builds; the density cutoffs (skip a block whose |dP| is below them, whatever its
         integrals) cost the most accuracy: on (H2O)64 / cc-pVTZ a difference build at the
         full-build cutoffs is off by ~1e-7 Eh, with the density tests off by ~2e-8 Eh.
-        FP64 delta builds use delta_scale (default 1e-3) times both cutoffs, which keeps
-        the energy drift over the final iterations below the 1e-9 Eh convergence test
-        (1e-2 does not).  In the mixed phase, whose errors the full FP64 build at the
-        precision switch removes, the Schwarz cutoff is mixed_delta_scale (default 10)
-        times the full-build one and the density cutoffs mixed_delta_density_scale
-        (default 1e-3) times theirs: ~5e-8 Eh per build at half the cost."""
-        if mode == "delta_mixed":
-            return self.jk_scaled(P, mode, scale * self.mixed_delta_scale, scale * self.mixed_delta_density_scale)
-        if mode == "delta":
-            return self.jk_scaled(P, mode, scale * self.delta_scale, scale * self.delta_scale)
+        FP64 delta builds use the Schwarz cutoff delta_cutoff (default 1e-14) and the
+        density cutoffs delta_density_cutoff (default 1e-12), which keeps the energy drift
+        over the final iterations below the 1e-9 Eh convergence test (a Schwarz cutoff of
+        1e-13 or 1e-12 leaves ~2e-9 Eh per iteration).  In the mixed phase, whose errors
+        the full FP64 build at the precision switch removes, the Schwarz cutoff is
+        mixed_delta_cutoff (default 1e-10) and the density cutoffs
+        mixed_delta_density_cutoff (default 1e-12): ~5e-8 Eh per build on
+        (H2O)64 / cc-pVTZ at half the cost of the FP64 ones.  Cutoffs are never loosened
+        below the full-build ones."""
+        if mode in ("delta", "delta_mixed"):
+            s, j, _, _ = self.engine.cutoffs
+            cs, cd = ((self.mixed_delta_cutoff, self.mixed_delta_density_cutoff) if mode == "delta_mixed"
+                      else (self.delta_cutoff, self.delta_density_cutoff))
+            return self.jk_scaled(P, mode, min(1.0, cs / s) if s > 0 else 1.0,
+                                  min(1.0, cd / j) if j > 0 else 1.0)
         return self.jk(P, mode, scale)
 
     def jk_scaled(self, P, mode, schwarz_scale, density_scale):
@@ -140,7 +148,13 @@ class GPUSCF:
 
     def run(self, precision="fp64", mixed_until=1e-4, incremental=False, guess="promolecule",
             log=None, guess_model=None, **kw):
-        """RHF to convergence; the wall time includes making the initial guess."""
+        """RHF to convergence; the wall time includes making the initial guess.  A mixed
+        run makes its full builds at DIIS error >= 1e-2 (the first one included) with the
+        Schwarz cutoff loose_cutoff (1e-8; ~1e-5 in F, 2.2x faster builds on
+        (H2O)64 / cc-pVTZ) unless loose_scale is given."""
+        if precision == "mixed" and "loose_scale" not in kw:
+            s = self.engine.cutoffs[0]
+            kw["loose_scale"] = max(1.0, self.loose_cutoff / s) if s > 0 else 1.0
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         P0 = self.guess(guess, guess_model)
