@@ -99,18 +99,30 @@ __device__ __forceinline__ void rys_root_one(double X, int r, double &u, double 
     const int k = (int)(X * (1.0 / RYS_H));
     const double t = (X - k * RYS_H) * (2.0 / RYS_H) - 1.0, tt = 2.0 * t;
     const double *c = rys_table_data + rys_table_offset[N] + k * (RYS_NCHEB * 2 * N);
+    // the coefficients are loaded half a chain at a time, each half before the
+    // recurrence uses any of it, so their latencies overlap instead of adding up
+    constexpr int H = RYS_NCHEB / 2;
+    double cu[H], cw[H];
     double bu1 = 0.0, bu2 = 0.0, bw1 = 0.0, bw2 = 0.0;
 #pragma unroll
-    for (int m = RYS_NCHEB - 1; m >= 1; --m) {
-        const double bu0 = tt * bu1 - bu2 + c[m * 2 * N + r];
-        const double bw0 = tt * bw1 - bw2 + c[m * 2 * N + N + r];
-        bu2 = bu1;
-        bu1 = bu0;
-        bw2 = bw1;
-        bw1 = bw0;
+    for (int h = 1; h >= 0; --h) {
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            cu[i] = c[(h * H + i) * 2 * N + r];
+            cw[i] = c[(h * H + i) * 2 * N + N + r];
+        }
+#pragma unroll
+        for (int i = H - 1; i >= (h ? 0 : 1); --i) {
+            const double bu0 = tt * bu1 - bu2 + cu[i];
+            const double bw0 = tt * bw1 - bw2 + cw[i];
+            bu2 = bu1;
+            bu1 = bu0;
+            bw2 = bw1;
+            bw1 = bw0;
+        }
     }
-    u = t * bu1 - bu2 + c[r];
-    w = t * bw1 - bw2 + c[N + r];
+    u = t * bu1 - bu2 + cu[0];
+    w = t * bw1 - bw2 + cw[0];
 }
 
 // Same evaluation with the n-root coefficient block passed in (LDS copy or global):
@@ -871,12 +883,27 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         const double f = t.f;
         const int fr = frs >> 16, fs = frs & 0xffff;
-        // stage the ket-dependent density blocks
-        for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
-        for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fs + u / NQ) * ld + fq + u % NQ];
-        for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fr + u / NQ) * ld + fq + u % NQ];
-        for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fs + u / NP) * ld + fp + u % NP];
-        for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fr + u / NP) * ld + fp + u % NP];
+        // stage the ket-dependent density blocks; when each fits one pass of the 64 lanes
+        // the five loads are issued together and waited for once
+        if constexpr (NRS <= 64 && NS * NQ <= 64 && NR * NQ <= 64 && NS * NP <= 64 && NR * NP <= 64) {
+            const int u = lane;
+            const double v0 = u < NRS ? P[(size_t)(fr + u / NS) * ld + fs + u % NS] : 0.0;
+            const double v1 = u < NS * NQ ? P[(size_t)(fs + u / NQ) * ld + fq + u % NQ] : 0.0;
+            const double v2 = u < NR * NQ ? P[(size_t)(fr + u / NQ) * ld + fq + u % NQ] : 0.0;
+            const double v3 = u < NS * NP ? P[(size_t)(fs + u / NP) * ld + fp + u % NP] : 0.0;
+            const double v4 = u < NR * NP ? P[(size_t)(fr + u / NP) * ld + fp + u % NP] : 0.0;
+            if (u < NRS) L.Prs[u] = v0;
+            if (u < NS * NQ) L.Psq[u] = v1;
+            if (u < NR * NQ) L.Prq[u] = v2;
+            if (u < NS * NP) L.Psp[u] = v3;
+            if (u < NR * NP) L.Prp[u] = v4;
+        } else {
+            for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
+            for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fs + u / NQ) * ld + fq + u % NQ];
+            for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fr + u / NQ) * ld + fq + u % NQ];
+            for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fs + u / NP) * ld + fp + u % NP];
+            for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fr + u / NP) * ld + fp + u % NP];
+        }
         const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L, gbuf);
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
