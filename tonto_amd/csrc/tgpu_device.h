@@ -488,20 +488,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         if (!t.skipK) {
             // K(p,r) += I P(s,q); K(p,s) += I P(r,q); K(q,r) += I P(s,p); K(q,s) += I P(r,p)
             // (shell1quartet.foo:11508-11521)
+            // P is symmetric: P(s,q) etc. are read as P(q,s) from the bra pair's rows,
+            // which every lane of the wavefront shares (cache-resident), instead of one
+            // row per lane
             double Psq[NS * NQ], Prq[NR * NQ], Psp[NS * NP], Prp[NR * NP];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fs + s) * ld + fq + q];
+                for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fq + q) * ld + fs + s];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fr + r) * ld + fq + q];
+                for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fq + q) * ld + fr + r];
             }
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fs + s) * ld + fp + p];
+                for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fp + p) * ld + fs + s];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fr + r) * ld + fp + p];
+                for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fp + p) * ld + fr + r];
             }
 #pragma unroll
             for (int p = 0; p < NP; ++p)
@@ -883,15 +886,16 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         const double f = t.f;
         const int fr = frs >> 16, fs = frs & 0xffff;
-        // stage the ket-dependent density blocks; when each fits one pass of the 64 lanes
-        // the five loads are issued together and waited for once
+        // stage the ket-dependent density blocks (P symmetric: the K-side blocks come from
+        // the bra pair's rows); when each fits one pass of the 64 lanes the five loads are
+        // issued together and waited for once
         if constexpr (NRS <= 64 && NS * NQ <= 64 && NR * NQ <= 64 && NS * NP <= 64 && NR * NP <= 64) {
             const int u = lane;
             const double v0 = u < NRS ? P[(size_t)(fr + u / NS) * ld + fs + u % NS] : 0.0;
-            const double v1 = u < NS * NQ ? P[(size_t)(fs + u / NQ) * ld + fq + u % NQ] : 0.0;
-            const double v2 = u < NR * NQ ? P[(size_t)(fr + u / NQ) * ld + fq + u % NQ] : 0.0;
-            const double v3 = u < NS * NP ? P[(size_t)(fs + u / NP) * ld + fp + u % NP] : 0.0;
-            const double v4 = u < NR * NP ? P[(size_t)(fr + u / NP) * ld + fp + u % NP] : 0.0;
+            const double v1 = u < NS * NQ ? P[(size_t)(fq + u % NQ) * ld + fs + u / NQ] : 0.0;
+            const double v2 = u < NR * NQ ? P[(size_t)(fq + u % NQ) * ld + fr + u / NQ] : 0.0;
+            const double v3 = u < NS * NP ? P[(size_t)(fp + u % NP) * ld + fs + u / NP] : 0.0;
+            const double v4 = u < NR * NP ? P[(size_t)(fp + u % NP) * ld + fr + u / NP] : 0.0;
             if (u < NRS) L.Prs[u] = v0;
             if (u < NS * NQ) L.Psq[u] = v1;
             if (u < NR * NQ) L.Prq[u] = v2;
@@ -899,10 +903,10 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
             if (u < NR * NP) L.Prp[u] = v4;
         } else {
             for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
-            for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fs + u / NQ) * ld + fq + u % NQ];
-            for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fr + u / NQ) * ld + fq + u % NQ];
-            for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fs + u / NP) * ld + fp + u % NP];
-            for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fr + u / NP) * ld + fp + u % NP];
+            for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fq + u % NQ) * ld + fs + u / NQ];
+            for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fq + u % NQ) * ld + fr + u / NQ];
+            for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fp + u % NP) * ld + fs + u / NP];
+            for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fp + u % NP) * ld + fr + u / NP];
         }
         const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L, gbuf);
         ++nq;
