@@ -107,9 +107,10 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     diis_reset_on_switch: drop the DIIS history at the precision switch.
     incremental: update the 2e Fock part with builds on P - P_prev; after rebuild_every
       builds since the last full one, at the precision switch and once converged a full
-      build is done, and convergence is only accepted on a full FP64 build.  While the DIIS error is still at
-      or above incremental_from, P - P_prev is as large as P and a difference build (run
-      at a tighter cutoff) would cost more than a full one, so full builds are used.
+      build is done, and convergence is only accepted on a full FP64 build.  While the
+      DIIS error is still at or above incremental_from, P - P_prev is as large as P and a
+      difference build (run at a tighter cutoff) would cost more than a full one, so full
+      builds are used.
     """
     dt = h.dtype
     n_occ = n_electrons // 2
