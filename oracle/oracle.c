@@ -412,12 +412,32 @@ static void max_density(const osys_t *s, const double *P, double *Pmax)
         }
 }
 
+/* UHF: max(|Pa|, |Pb|) per shell pair (molecule.fock.foo:3670-3701) */
+static void max_density2(const osys_t *s, const double *Pa, const double *Pb, double *Pmax)
+{
+    int ns = s->n_shell, n = s->n_bf;
+    for (int a = 0; a < ns; a++)
+        for (int b = 0; b <= a; b++) {
+            double m = 0.0;
+            for (int i = 0; i < ncomp(s->l[a]); i++)
+                for (int j = 0; j < ncomp(s->l[b]); j++) {
+                    size_t e = (size_t)(s->first_bf[a] + i) * n + s->first_bf[b] + j;
+                    double v = fabs(Pa[e]), w = fabs(Pb[e]);
+                    if (w > v) v = w;
+                    if (v > m) m = v;
+                }
+            Pmax[a * (a + 1) / 2 + b] = m;
+        }
+}
+
 static inline int pidx(int x, int y) { return x >= y ? x * (x + 1) / 2 + y : y * (y + 1) / 2 + x; }
 
 typedef struct {
     const osys_t *s;
     const ocut_t *cut;
     const double *P, *max_I, *Pmax;
+    const double *Pb;             /* UHF: the B-spin density (NULL for RHF) */
+    double **Jbt, **Kbt;          /* UHF: per-worker B-spin J and K */
     double P_max, IP_max;
     double **Jt, **Kt, **Ibuf;
     pairdat_t **ws;
@@ -430,9 +450,10 @@ static void jk_item(void *arg, long ab, int w)
     jk_job *jb = (jk_job *)arg;
     const osys_t *s = jb->s;
     const ocut_t *cut = jb->cut;
-    const double *max_I = jb->max_I, *Pmax = jb->Pmax, *P = jb->P;
+    const double *max_I = jb->max_I, *Pmax = jb->Pmax, *P = jb->P, *Pb = jb->Pb;
     int n = s->n_bf;
     double *Jl = jb->Jt[w], *Kl = jb->Kt[w], *I = jb->Ibuf[w];
+    double *Jbl = Pb ? jb->Jbt[w] : NULL, *Kbl = Pb ? jb->Kbt[w] : NULL;
     long *st = jb->st[w];
     if (max_I[ab] * jb->IP_max < cut->schwarz) return;
     int a = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
@@ -476,12 +497,27 @@ static void jk_item(void *arg, long ab, int w)
                         if (!skip_J) {
                             Jl[(size_t)i * n + j] += v * P[(size_t)k * n + l];
                             Jl[(size_t)k * n + l] += v * P[(size_t)i * n + j];
+                            if (Pb) {
+                                /* UHF (shell1quartet.foo:11530-11714): B-spin bra side into JB;
+                                 * the B-spin ket side goes into JA, because the engine points
+                                 * JBcd at JA(cd) (molecule.fock.foo:2637), except for ab = cd,
+                                 * where Tonto doubles the bra update of JB instead */
+                                Jbl[(size_t)i * n + j] += v * Pb[(size_t)k * n + l];
+                                if (a == c && b == d) Jbl[(size_t)k * n + l] += v * Pb[(size_t)i * n + j];
+                                else Jl[(size_t)k * n + l] += v * Pb[(size_t)i * n + j];
+                            }
                         }
                         if (!skip_K) {
                             Kl[(size_t)i * n + k] += v * P[(size_t)l * n + j];
                             Kl[(size_t)i * n + l] += v * P[(size_t)k * n + j];
                             Kl[(size_t)j * n + k] += v * P[(size_t)l * n + i];
                             Kl[(size_t)j * n + l] += v * P[(size_t)k * n + i];
+                            if (Pb) {
+                                Kbl[(size_t)i * n + k] += v * Pb[(size_t)l * n + j];
+                                Kbl[(size_t)i * n + l] += v * Pb[(size_t)k * n + j];
+                                Kbl[(size_t)j * n + k] += v * Pb[(size_t)l * n + i];
+                                Kbl[(size_t)j * n + l] += v * Pb[(size_t)k * n + i];
+                            }
                         }
                     }
     }
@@ -498,8 +534,11 @@ static void jk_item_sample(void *arg, long item, int w)
  * [3] K-skips, [4] quartets using skipped primitive-pair lists */
 static void jk_item_sample(void *arg, long item, int w);
 
+static void post_jk(const osys_t *s, double *J, double *K);
+
 static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
-                           double *J, double *K, long *stats, long sample_stride)
+                           double *J, double *K, long *stats, long sample_stride,
+                           const double *Pb, double *Jb, double *Kb)
 {
     comp_tables();
     int ns = s->n_shell, n = s->n_bf;
@@ -508,7 +547,8 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
     if (max_I_in) memcpy(max_I, max_I_in, sizeof(double) * npair);
     else orc_schwarz(s, cut->rys_quad, max_I);
     double *Pmax = malloc(sizeof(double) * npair);
-    max_density(s, P, Pmax);
+    if (Pb) max_density2(s, P, Pb, Pmax);
+    else max_density(s, P, Pmax);
     double P_max = 0.0, I_max = 0.0;
     for (long i = 0; i < npair; i++) {
         if (Pmax[i] > P_max) P_max = Pmax[i];
@@ -517,15 +557,20 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
     int nthr = orc_max_threads();
     jk_job jb;
     jb.s = s; jb.cut = cut; jb.P = P; jb.max_I = max_I; jb.Pmax = Pmax;
+    jb.Pb = Pb;
     jb.P_max = P_max; jb.IP_max = I_max * P_max;
     jb.Jt = malloc(sizeof(double *) * nthr);
     jb.Kt = malloc(sizeof(double *) * nthr);
+    jb.Jbt = malloc(sizeof(double *) * nthr);
+    jb.Kbt = malloc(sizeof(double *) * nthr);
     jb.Ibuf = malloc(sizeof(double *) * nthr);
     jb.ws = malloc(sizeof(pairdat_t *) * nthr);
     jb.st = calloc(nthr, sizeof(long[5]));
     for (int t = 0; t < nthr; t++) {
         jb.Jt[t] = calloc((size_t)n * n, sizeof(double));
         jb.Kt[t] = calloc((size_t)n * n, sizeof(double));
+        jb.Jbt[t] = Pb ? calloc((size_t)n * n, sizeof(double)) : NULL;
+        jb.Kbt[t] = Pb ? calloc((size_t)n * n, sizeof(double)) : NULL;
         jb.Ibuf[t] = malloc(sizeof(double) * 10 * 10 * 10 * 10);
         jb.ws[t] = malloc(2 * sizeof(pairdat_t));
     }
@@ -538,15 +583,32 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
     }
     memset(J, 0, sizeof(double) * n * n);
     memset(K, 0, sizeof(double) * n * n);
+    if (Pb) {
+        memset(Jb, 0, sizeof(double) * n * n);
+        memset(Kb, 0, sizeof(double) * n * n);
+    }
     long st[5] = {0, 0, 0, 0, 0};
     for (int t = 0; t < nthr; t++) {
         for (long i = 0; i < (long)n * n; i++) { J[i] += jb.Jt[t][i]; K[i] += jb.Kt[t][i]; }
+        if (Pb)
+            for (long i = 0; i < (long)n * n; i++) { Jb[i] += jb.Jbt[t][i]; Kb[i] += jb.Kbt[t][i]; }
         for (int k = 0; k < 5; k++) st[k] += jb.st[t][k];
         free(jb.Jt[t]); free(jb.Kt[t]); free(jb.Ibuf[t]); free(jb.ws[t]);
+        free(jb.Jbt[t]); free(jb.Kbt[t]);
     }
-    free(jb.Jt); free(jb.Kt); free(jb.Ibuf); free(jb.ws); free(jb.st);
-    /* post-processing: weight_diagonal_blocks(J,2); blockwise_symmetric_fold(K); J = 2J;
-     * symmetric_reflect (lower -> upper) */
+    free(jb.Jt); free(jb.Kt); free(jb.Ibuf); free(jb.ws); free(jb.st); free(jb.Jbt); free(jb.Kbt);
+    post_jk(s, J, K);
+    if (Pb) post_jk(s, Jb, Kb);
+    if (stats) memcpy(stats, st, sizeof st);
+    free(max_I); free(Pmax);
+}
+
+/* post-processing (molecule.fock.foo:1485-1500, 2688-2706 for UHF):
+ * weight_diagonal_blocks(J,2); blockwise_symmetric_fold(K); J = 2J; symmetric_reflect
+ * (lower -> upper) */
+static void post_jk(const osys_t *s, double *J, double *K)
+{
+    int ns = s->n_shell, n = s->n_bf;
     for (int a = 0; a < ns; a++) {
         int fa = s->first_bf[a], na = ncomp(s->l[a]);
         for (int i = 0; i < na; i++)
@@ -568,8 +630,6 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
             J[(size_t)j * n + i] = J[(size_t)i * n + j];
             K[(size_t)j * n + i] = K[(size_t)i * n + j];
         }
-    if (stats) memcpy(stats, st, sizeof st);
-    free(max_I); free(Pmax);
 }
 
 /* ---------------- one-electron integrals (molecule.ints.foo:34-415) ------------ */
@@ -696,7 +756,15 @@ void orc_one_electron(const osys_t *s, double *S, double *T, double *V)
 void orc_make_r_JK(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
                    double *J, double *K, long *stats)
 {
-    make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, 1);
+    make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, 1, NULL, NULL, NULL);
+}
+
+/* UHF J/K (make_u_JK_engine, molecule.fock.foo:2558-2712): Ja, Jb, Ka, Kb as Tonto
+ * returns them (see jk_item for the J split). */
+void orc_make_u_JK(const osys_t *s, const ocut_t *cut, const double *Pa, const double *Pb,
+                   const double *max_I_in, double *Ja, double *Jb, double *Ka, double *Kb, long *stats)
+{
+    make_r_JK_impl(s, cut, Pa, max_I_in, Ja, Ka, stats, 1, Pb, Jb, Kb);
 }
 
 /* Timing sample for the CPU baseline: the same build restricted to every stride-th
@@ -704,5 +772,5 @@ void orc_make_r_JK(const osys_t *s, const ocut_t *cut, const double *P, const do
 void orc_make_r_JK_sample(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
                           double *J, double *K, long *stats, long stride)
 {
-    make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, stride < 1 ? 1 : stride);
+    make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, stride < 1 ? 1 : stride, NULL, NULL, NULL);
 }

@@ -28,6 +28,8 @@ def lib():
         _lib.orc_make_r_JK.argtypes = [ctypes.c_void_p, ctypes.c_void_p, d, ctypes.c_void_p, d, d,
                                        ctypes.POINTER(ctypes.c_long)]
         _lib.orc_make_r_JK_sample.argtypes = _lib.orc_make_r_JK.argtypes + [ctypes.c_long]
+        _lib.orc_make_u_JK.argtypes = [ctypes.c_void_p, ctypes.c_void_p, d, d, ctypes.c_void_p, d, d, d, d,
+                                       ctypes.POINTER(ctypes.c_long)]
         _lib.orc_eri_quartet.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + \
             [ctypes.c_double, ctypes.c_int, d]
         _lib.orc_eri_quartet.restype = ctypes.c_long
@@ -98,6 +100,24 @@ class OracleSystem:
             return J, K, dict(quartets=st[0], prim_quartets=st[1], skip_J=st[2], skip_K=st[3],
                               skip_lists=st[4])
         return J, K
+
+    def make_u_JK(self, Pa, Pb, schwarz=1e-9, jp=1e-9, kp=1e-9, pair=1e-6, max_I=None, return_stats=False):
+        """UHF J/K as Tonto's make_u_JK_engine returns them (molecule.fock.foo:2558-2712):
+        (Ja, Jb, Ka, Kb); Ja + Jb = J(Pa + Pb), Ka = K(Pa), Kb = K(Pb), with the J split
+        of the engine's JBcd => JA(cd) aliasing (:2637) and max(|Pa|, |Pb|) screening."""
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        n = self.n_bf
+        out = [np.zeros((n, n)) for _ in range(4)]
+        cut = _OCut(schwarz, jp, kp, pair, 0)
+        st = (ctypes.c_long * 5)()
+        mi = None if max_I is None else np.ascontiguousarray(max_I, dtype=np.float64)
+        lib().orc_make_u_JK(ctypes.byref(self.s), ctypes.byref(cut), _dp(Pa), _dp(Pb),
+                            None if mi is None else mi.ctypes.data, *[_dp(x) for x in out], st)
+        if return_stats:
+            return (*out, dict(quartets=st[0], prim_quartets=st[1], skip_J=st[2], skip_K=st[3],
+                               skip_lists=st[4]))
+        return tuple(out)
 
     def make_r_JK_sample(self, P, stride, schwarz=1e-9, jp=1e-9, kp=1e-9, pair=1e-6, max_I=None):
         """Timing sample: the build over every stride-th (ab| pair; returns stats."""
