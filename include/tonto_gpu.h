@@ -126,6 +126,21 @@ int tgpu_make_r_JK(tgpu_ctx *ctx, const double *P, double *J, double *K, int fla
  * stream.  No allocation; P must stay valid until the stream reaches this work. */
 int tgpu_make_r_JK_device(tgpu_ctx *ctx, const double *P, double *J, double *K, int flags);
 
+/* UHF J and K from the alpha and beta densities Pa, Pb, host arrays of n_bf*n_bf
+ * doubles (MOLECULE.FOCK:make_u_JK_engine, molecule.fock.foo:2558-2712): Ja, Jb, Ka, Kb
+ * after the same post-processing as the RHF build (2688-2706).  As in Tonto, screening
+ * uses max(|Pa|, |Pb|) per shell pair (3670-3701), Ka = K(Pa), Kb = K(Pb), and
+ * Ja + Jb = J(Pa + Pb) with Tonto's split between them (the engine adds the beta
+ * density's ket-side J into Ja, JBcd => JA(cd), :2637).  flags: 0 or TGPU_JK_MIXED.
+ * Synchronous; the statistics afterwards are those of the engine's quartets. */
+int tgpu_make_u_JK(tgpu_ctx *ctx, const double *Pa, const double *Pb, double *Ja, double *Jb,
+                   double *Ka, double *Kb, int flags);
+
+/* Same on device pointers, asynchronous on the context's stream.  Three passes of the
+ * build; allocates two n_bf*n_bf scratch matrices on its first call. */
+int tgpu_make_u_JK_device(tgpu_ctx *ctx, const double *Pa, const double *Pb, double *Ja, double *Jb,
+                          double *Ka, double *Kb, int flags);
+
 /* Statistics of the last build (synchronises the stream). */
 int tgpu_get_stats(tgpu_ctx *ctx, tgpu_stats *stats);
 

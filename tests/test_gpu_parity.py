@@ -208,6 +208,37 @@ def test_mixed_incremental_scf_energy(torch_cuda):
     assert rmx.n_builds["delta"] + rmx.n_builds["delta_mixed"] > 0
 
 
+@pytest.mark.parametrize("basis,cut,krows", [("6-31G(d)", 1e-9, False), ("6-31G(d)", 1e-12, True),
+                                              ("cc-pVTZ", 1e-11, False)])
+def test_uhf_against_oracle(torch_cuda, oracle_mod, basis, cut, krows):
+    """UHF J/K (make_u_JK_engine) against the oracle: Ja, Jb, Ka, Kb to 1e-10 and the
+    same quartets, with two different random densities; cc-pVTZ runs the f-shell
+    wavefront classes, krows the lane kernels' LDS K-row form."""
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(2 if basis == "cc-pVTZ" else 4, basis, seed=5)
+    old = os.environ.get("TGPU_KROW_MIN_KETS")
+    if krows:
+        os.environ["TGPU_KROW_MIN_KETS"] = "0"
+    try:
+        e = _engine(mol, schwarz=cut)
+    finally:
+        if old is None:
+            os.environ.pop("TGPU_KROW_MIN_KETS", None)
+        else:
+            os.environ["TGPU_KROW_MIN_KETS"] = old
+    mi = e.schwarz()
+    o = oracle_mod.OracleSystem(mol)
+    Pa = _rand_density(mol.n_bf, 21)
+    Pb = _rand_density(mol.n_bf, 22, 0.7)
+    got = e.make_u_JK(Pa, Pb)
+    st = e.stats()
+    *ref, so = o.make_u_JK(Pa, Pb, schwarz=cut, max_I=mi, return_stats=True)
+    d = [float(np.max(np.abs(x - y))) for x, y in zip(got, ref)]
+    print(f"{basis} n_bf={mol.n_bf}: dJa dJb dKa dKb = {d}, quartets gpu={st['quartets']} oracle={so['quartets']}")
+    assert st["quartets"] == so["quartets"] and st["prim_quartets"] == so["prim_quartets"]
+    assert max(d) <= 1e-10
+
+
 def _uneven_batch(n_mol=6, basis="6-31G(d,p)", seed=7):
     from tonto_amd.geometry import water_cluster
     rng = np.random.default_rng(seed)

@@ -264,6 +264,10 @@ struct JKArgs {
     int krow_rows;   // KROWS form: the first krow_rows K rows of the bra pair live in LDS
     int krow_budget; // cap on a work-group's LDS in the KROWS form (bytes; 0: default)
     int krow_min_kets;  // shortest ket segment (n_ket_avg) for the KROWS form
+    // UHF pass: J from Tonto's ket side only, X(cd) += f (ab|cd) P(ab) with ab the pair of
+    // larger Tonto index, for ab != cd (the updates Tonto's UHF engine puts into JA
+    // through JBcd => JA(cd), molecule.fock.foo:2637)
+    int j_ket_nonsame;
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -464,25 +468,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
         const double f = t.f;
+        // which J sides: both, or in the UHF pass only Tonto's ket side (the pair of smaller
+        // Tonto index receives), none for ab = cd
+        bool j_ket = true, j_bra = true;
+        if (a.j_ket_nonsame) {
+            const int tk = ldc(a.pr.rec0 + t.sk).x;
+            j_ket = b.tb > tk;
+            j_bra = b.tb < tk;
+        }
         if (!t.skipJ) {
             double Prs[NRS];
 #pragma unroll
             for (int r = 0; r < NR; ++r)
 #pragma unroll
                 for (int s = 0; s < NS; ++s) Prs[r * NS + s] = P[(size_t)(fr + r) * ld + fs + s];
+            if (j_ket) {
 #pragma unroll
-            for (int rs = 0; rs < NRS; ++rs) {
-                double acc = 0.0;
+                for (int rs = 0; rs < NRS; ++rs) {
+                    double acc = 0.0;
 #pragma unroll
-                for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + rs] * Ppq[pq];
-                unsafeAtomicAdd(&b.X[(size_t)(fr + rs / NS) * ld + fs + rs % NS], f * acc);
+                    for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + rs] * Ppq[pq];
+                    unsafeAtomicAdd(&b.X[(size_t)(fr + rs / NS) * ld + fs + rs % NS], f * acc);
+                }
             }
+            if (j_bra) {
 #pragma unroll
-            for (int pq = 0; pq < NPQ; ++pq) {
-                double acc = 0.0;
+                for (int pq = 0; pq < NPQ; ++pq) {
+                    double acc = 0.0;
 #pragma unroll
-                for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * Prs[rs];
-                Jpq[pq] += f * acc;
+                    for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * Prs[rs];
+                    Jpq[pq] += f * acc;
+                }
             }
         }
         if (!t.skipK) {
@@ -911,21 +927,32 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L, gbuf);
         ++nq;
         npq += (unsigned long long)(bv.n * kv.n);
+        // J sides as in the lane form (both, or the UHF pass's Tonto ket side only)
+        bool j_ket = true, j_bra = true;
+        if (a.j_ket_nonsame) {
+            const int tk = ldc(a.pr.rec0 + t.sk).x;
+            j_ket = b.tb > tk;
+            j_bra = b.tb < tk;
+        }
         if (!t.skipJ) {
-            for (int u = lane; u < NRS; u += 64) {
-                double acc = 0.0;
-#pragma unroll 4
-                for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + u] * L.Ppq[pq];
-                unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
-            }
-#pragma unroll
-            for (int m = 0; m < MJ; ++m) {
-                const int pq = lane + 64 * m;
-                if (pq < NPQ) {
+            if (j_ket) {
+                for (int u = lane; u < NRS; u += 64) {
                     double acc = 0.0;
 #pragma unroll 4
-                    for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * L.Prs[rs];
-                    Jpq[m] += f * acc;
+                    for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + u] * L.Ppq[pq];
+                    unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
+                }
+            }
+            if (j_bra) {
+#pragma unroll
+                for (int m = 0; m < MJ; ++m) {
+                    const int pq = lane + 64 * m;
+                    if (pq < NPQ) {
+                        double acc = 0.0;
+#pragma unroll 4
+                        for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * L.Prs[rs];
+                        Jpq[m] += f * acc;
+                    }
                 }
             }
         }

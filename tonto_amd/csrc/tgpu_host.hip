@@ -72,8 +72,10 @@ __device__ __forceinline__ void decode_pair(long long ab, int &x, int &y)
     y = (int)(ab - (long long)x * (x + 1) / 2);
 }
 
-// P_max(ab) = max |P(a-block, b-block)| per Tonto shell pair (molecule.fock.foo:3601-3633)
-__global__ void pmax_kernel(const double *__restrict__ P, BatchArgs b, double *__restrict__ Pmax)
+// P_max(ab) = max |P(a-block, b-block)| per Tonto shell pair (molecule.fock.foo:3601-3633);
+// with a second density P2 (UHF) the maximum over |P| and |P2| (3670-3701)
+__global__ void pmax_kernel(const double *__restrict__ P, const double *__restrict__ P2, BatchArgs b,
+                            double *__restrict__ Pmax)
 {
     const long long npair = b.pair_off[b.n_mol];
     for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < npair;
@@ -88,6 +90,11 @@ __global__ void pmax_kernel(const double *__restrict__ P, BatchArgs b, double *_
         double v = 0.0;
         for (int i = 0; i < na; ++i)
             for (int j = 0; j < nb; ++j) v = fmax(v, fabs(Pm[(size_t)(fa + i) * ld + fb + j]));
+        if (P2) {
+            const double *P2m = P2 + b.mat_off[m];
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j) v = fmax(v, fabs(P2m[(size_t)(fa + i) * ld + fb + j]));
+        }
         Pmax[g] = v;
     }
 }
@@ -305,6 +312,7 @@ struct tgpu_ctx {
     DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol;
     DBuf<PrimRec> d_prim;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
+    DBuf<double> d_hP2, d_hJ2, d_hK2, d_uJ, d_uK;  // UHF: host-entry staging, ket-side pass
     DBuf<unsigned long long> d_counters;
     // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
     DBuf<double> d_ws;
@@ -680,6 +688,7 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_prim);
     dfree(c->d_X); dfree(c->d_Y);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
+    dfree(c->d_hP2); dfree(c->d_hJ2); dfree(c->d_hK2); dfree(c->d_uJ); dfree(c->d_uK);
     dfree(c->d_counters);
     dfree(c->d_ws);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -843,16 +852,18 @@ static const jk_launcher kLaunch[NQCLS] = {
 };
 static int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
 
-// Enqueue one complete build on stream s (no host synchronisation).
+// Enqueue one complete build on stream s (no host synchronisation).  UHF passes: P2 is the
+// other spin's density (screening by max(|P|, |P2|)), j_ket_nonsame selects the J of
+// Tonto's ket side only (and no K).
 static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, int flags,
-                          hipStream_t s, bool timing)
+                          hipStream_t s, bool timing, const double *P2 = nullptr, int j_ket_nonsame = 0)
 {
     const bool mixed = flags & TGPU_JK_MIXED;
     const BatchArgs ba = batch_args(c);
     fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_X.p, c->n_elem);
     fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_Y.p, c->n_elem);
     fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(c->d_counters.p, 3 * NQCLS);
-    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(P, ba, c->d_Pmax.p);
+    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(P, P2, ba, c->d_Pmax.p);
     mol_max_kernel<<<c->n_mol, 256, 0, s>>>(c->d_Pmax.p, ba, c->d_maxI_mol.p, c->d_scal.p);
     if (timing) (void)hipEventRecord(c->ev0, s);
     JKArgs a;
@@ -869,6 +880,8 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     // in place of the density cutoff, and the other matrix is never digested
     if (flags & TGPU_J_ONLY) { a.jp = 1e-20; a.kp = HUGE_VAL; }
     if (flags & TGPU_K_ONLY) { a.kp = 1e-20; a.jp = HUGE_VAL; }
+    a.j_ket_nonsame = j_ket_nonsame;
+    if (j_ket_nonsame) a.kp = HUGE_VAL;  // the pass digests no K
     a.fp32_thr = c->fp32_thr;
     a.X = c->d_X.p;
     a.Y = c->d_Y.p;
@@ -964,6 +977,66 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
     HIPCHK(c, hipEventRecord(c->ev_out, c->gstream));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     c->timed = false;
+    return TGPU_OK;
+}
+
+__global__ void uhf_combine_kernel(double *__restrict__ Ja, double *__restrict__ Jb,
+                                   const double *__restrict__ Jk, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        Ja[i] += Jk[i];
+        Jb[i] -= Jk[i];
+    }
+}
+
+// UHF J/K (make_u_JK_engine, molecule.fock.foo:2558-2712) as three passes of the RHF
+// kernels, all screened by max(|Pa|, |Pb|): the Pb pass restricted to Tonto's ket side
+// (Jk), the Pb pass (J(Pb), Kb) and the Pa pass (J(Pa), Ka), then Ja = J(Pa) + Jk and
+// Jb = J(Pb) - Jk.  Run last, the Pa pass leaves the statistics (its quartets are the
+// engine's).
+extern "C" int tgpu_make_u_JK_device(tgpu_ctx *c, const double *Pa, const double *Pb, double *Ja, double *Jb,
+                                     double *Ka, double *Kb, int flags)
+{
+    if (!c || !Pa || !Pb || !Ja || !Jb || !Ka || !Kb) return TGPU_E_ARG;
+    if (flags & ~TGPU_JK_MIXED) {
+        c->err = "tgpu_make_u_JK: only TGPU_JK_MIXED is supported";
+        return TGPU_E_ARG;
+    }
+    if (!c->have_schwarz) {
+        int rc = tgpu_schwarz(c, nullptr);
+        if (rc) return rc;
+    }
+    const size_t nn = (size_t)c->n_elem;
+    int rc;
+    if ((rc = dalloc(c, c->d_uJ, nn)) || (rc = dalloc(c, c->d_uK, nn))) return rc;
+    enqueue_build(c, Pb, c->d_uJ.p, c->d_uK.p, flags, c->stream, false, Pa, 1);
+    enqueue_build(c, Pb, Jb, Kb, flags, c->stream, false, Pa, 0);
+    enqueue_build(c, Pa, Ja, Ka, flags, c->stream, true, Pb, 0);
+    uhf_combine_kernel<<<grid_for(nn), 256, 0, c->stream>>>(Ja, Jb, c->d_uJ.p, nn);
+    c->timed = true;
+    HIPCHK(c, hipGetLastError());
+    return TGPU_OK;
+}
+
+extern "C" int tgpu_make_u_JK(tgpu_ctx *c, const double *Pa, const double *Pb, double *Ja, double *Jb,
+                              double *Ka, double *Kb, int flags)
+{
+    if (!c || !Pa || !Pb || !Ja || !Jb || !Ka || !Kb) return TGPU_E_ARG;
+    const size_t nn = (size_t)c->n_elem;
+    int rc;
+    if ((rc = dalloc(c, c->d_hP, nn)) || (rc = dalloc(c, c->d_hJ, nn)) || (rc = dalloc(c, c->d_hK, nn)) ||
+        (rc = dalloc(c, c->d_hP2, nn)) || (rc = dalloc(c, c->d_hJ2, nn)) || (rc = dalloc(c, c->d_hK2, nn)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_hP.p, Pa, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_hP2.p, Pb, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = tgpu_make_u_JK_device(c, c->d_hP.p, c->d_hP2.p, c->d_hJ.p, c->d_hJ2.p, c->d_hK.p, c->d_hK2.p,
+                                    flags)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(Ja, c->d_hJ.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Jb, c->d_hJ2.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Ka, c->d_hK.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Kb, c->d_hK2.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return TGPU_OK;
 }
 

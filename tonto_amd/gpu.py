@@ -20,7 +20,8 @@ TGPU_K_ONLY = 8
 # exported symbols, in the order of include/tonto_gpu.h (plus the two extensions)
 EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_set_stream",
            "tgpu_set_cutoffs", "tgpu_set_fp32_threshold", "tgpu_schwarz", "tgpu_make_r_JK",
-           "tgpu_make_r_JK_device", "tgpu_get_stats", "tgpu_one_electron",
+           "tgpu_make_r_JK_device", "tgpu_make_u_JK", "tgpu_make_u_JK_device", "tgpu_get_stats",
+           "tgpu_one_electron",
            "tgpu_get_class_counters", "tgpu_n_class_pairs", "tgpu_set_profiling",
            "tgpu_get_class_times", "tgpu_create_batch", "tgpu_batch_layout", "tgpu_set_variant",
            "tgpu_get_variant", "tgpu_autotune"]
@@ -67,6 +68,8 @@ def lib():
         L.tgpu_schwarz.argtypes = [vp, vp]
         L.tgpu_make_r_JK.argtypes = [vp, dp, dp, dp, ctypes.c_int]
         L.tgpu_make_r_JK_device.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+        L.tgpu_make_u_JK.argtypes = [vp, dp, dp, dp, dp, dp, dp, ctypes.c_int]
+        L.tgpu_make_u_JK_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         L.tgpu_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.tgpu_one_electron.argtypes = [vp, dp, dp, dp]
         L.tgpu_get_class_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]
@@ -177,6 +180,18 @@ class JKEngine:
             return self.unpack(J), self.unpack(K)
         n = self.n_bf
         return J.reshape(n, n), K.reshape(n, n)
+
+    def make_u_JK(self, Pa, Pb, mixed=False):
+        """UHF: host arrays Pa, Pb in, (Ja, Jb, Ka, Kb) out (synchronous; one molecule)."""
+        assert self.n_mol == 1
+        n = self.n_bf
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64).ravel()
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64).ravel()
+        assert Pa.size == Pb.size == n * n
+        out = [np.zeros(n * n) for _ in range(4)]
+        self._check(lib().tgpu_make_u_JK(self._ctx, _dp(Pa), _dp(Pb), *[_dp(x) for x in out],
+                                         TGPU_JK_MIXED if mixed else 0))
+        return tuple(x.reshape(n, n) for x in out)
 
     def make_r_JK_device(self, P_ptr, J_ptr, K_ptr, mixed=False, graph=False):
         """Device pointers (e.g. torch tensor .data_ptr()), asynchronous on the stream."""
