@@ -51,6 +51,13 @@ module tonto_gpu_mod
          real(c_double), intent(out) :: J(*), K(*)
          integer(c_int), value :: flags
       end function
+      integer(c_int) function tgpu_make_u_JK(ctx, Pa, Pb, Ja, Jb, Ka, Kb, flags) bind(c, name="tgpu_make_u_JK")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in)  :: Pa(*), Pb(*)
+         real(c_double), intent(out) :: Ja(*), Jb(*), Ka(*), Kb(*)
+         integer(c_int), value :: flags
+      end function
    end interface
 
 contains
@@ -64,6 +71,17 @@ contains
       integer(c_int) :: rc
       rc = tgpu_make_r_JK(ctx, P, J, K, 0_c_int)
       if (rc /= TGPU_OK) stop "libtonto_gpu: make_r_JK failed (see tgpu_last_error)"
+   end subroutine
+
+   ! Replacement body for make_u_JK_engine(J,K,P) (molecule.fock.foo:2558-2712):
+   ! J.a, J.b, K.a, K.b as Tonto's engine leaves them.
+   subroutine gpu_make_u_JK_engine(ctx, Pa, Pb, Ja, Jb, Ka, Kb)
+      type(c_ptr), intent(in) :: ctx
+      real(c_double), intent(in)  :: Pa(:,:), Pb(:,:)
+      real(c_double), intent(out) :: Ja(:,:), Jb(:,:), Ka(:,:), Kb(:,:)
+      integer(c_int) :: rc
+      rc = tgpu_make_u_JK(ctx, Pa, Pb, Ja, Jb, Ka, Kb, 0_c_int)
+      if (rc /= TGPU_OK) stop "libtonto_gpu: make_u_JK failed (see tgpu_last_error)"
    end subroutine
 
 end module
