@@ -445,45 +445,81 @@ typedef struct {
     long sample_stride;
 } jk_job;
 
+/* Planted screening mistakes, for the test that shows the screening audit catches them
+ * (tests/test_oracle_audit.py); 0 (the default) everywhere else.
+ *   1: KP reads the (b,c) block twice instead of (b,c) and (b,d)
+ *   2: skipped primitive-pair lists also when a and c (or b and d) share an atom
+ *   3: JP is the smaller of the two pair maxima instead of the larger */
+static int g_plant = 0;
+void orc_set_plant(int k) { g_plant = k; }
+
+/* Tonto's row test for the bra pair (molecule.fock.foo:1412): 1 if the row survives */
+static int ab_survives(const jk_job *jb, long ab)
+{
+    return !(jb->max_I[ab] * jb->IP_max < jb->cut->schwarz);
+}
+
+/* Tonto uses the skipped primitive-pair lists only when no two of the four atoms
+ * coincide (molecule.fock.foo:1456-1472) */
+static int uses_skip_lists(const osys_t *s, int a, int b, int c, int d)
+{
+    const int A = s->atom[a], B = s->atom[b], C = s->atom[c], D = s->atom[d];
+    if (g_plant == 2) return !(A == B || C == D || A == D || B == C);
+    return !(A == B || C == D || A == C || A == D || B == C || B == D);
+}
+
+/* Tonto's tests of quartet (ab|cd) after the row test: the Schwarz test with P_max
+ * (molecule.fock.foo:1430) and Schwarz_test (3497-3536, triangular indices 3516-3521).
+ * Returns 0 if the quartet is dropped, else bit 0 = J digested, bit 1 = K digested. */
+static int screen_quartet(const jk_job *jb, long ab, long cd, int a, int b, int c, int d)
+{
+    const double *max_I = jb->max_I, *Pmax = jb->Pmax;
+    const ocut_t *cut = jb->cut;
+    if (max_I[ab] * max_I[cd] * jb->P_max < cut->schwarz) return 0;
+    double JP = Pmax[ab] > Pmax[cd] ? Pmax[ab] : Pmax[cd];
+    if (g_plant == 3) JP = Pmax[ab] < Pmax[cd] ? Pmax[ab] : Pmax[cd];
+    double KP = Pmax[pidx(a, c)];
+    if (Pmax[pidx(a, d)] > KP) KP = Pmax[pidx(a, d)];
+    if (Pmax[pidx(b, c)] > KP) KP = Pmax[pidx(b, c)];
+    const long bd = g_plant == 1 ? pidx(b, c) : pidx(b, d);
+    if (Pmax[bd] > KP) KP = Pmax[bd];
+    int skip_J = (JP < cut->jp) || (JP * max_I[ab] * max_I[cd] < cut->schwarz);
+    int skip_K = (KP < cut->kp) || (KP * max_I[ab] * max_I[cd] < cut->schwarz);
+    return (!skip_J) | ((!skip_K) << 1);
+}
+
+static void decode(long ab, int *a, int *b)
+{
+    int x = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
+    while (x * (x + 1) / 2 > ab) x--;
+    while ((x + 1) * (x + 2) / 2 <= ab) x++;
+    *a = x;
+    *b = (int)(ab - (long)x * (x + 1) / 2);
+}
+
 static void jk_item(void *arg, long ab, int w)
 {
     jk_job *jb = (jk_job *)arg;
     const osys_t *s = jb->s;
     const ocut_t *cut = jb->cut;
-    const double *max_I = jb->max_I, *Pmax = jb->Pmax, *P = jb->P, *Pb = jb->Pb;
+    const double *P = jb->P, *Pb = jb->Pb;
     int n = s->n_bf;
     double *Jl = jb->Jt[w], *Kl = jb->Kt[w], *I = jb->Ibuf[w];
     double *Jbl = Pb ? jb->Jbt[w] : NULL, *Kbl = Pb ? jb->Kbt[w] : NULL;
     long *st = jb->st[w];
-    if (max_I[ab] * jb->IP_max < cut->schwarz) return;
-    int a = (int)((sqrt(8.0 * ab + 1.0) - 1.0) / 2.0);
-    while (a * (a + 1) / 2 > ab) a--;
-    while ((a + 1) * (a + 2) / 2 <= ab) a++;
-    int b = (int)(ab - a * (a + 1) / 2);
-    int atom_a = s->atom[a], atom_b = s->atom[b];
-    int ab_same = atom_a == atom_b;
+    if (!ab_survives(jb, ab)) return;
+    int a, b;
+    decode(ab, &a, &b);
     for (long cd = 0; cd <= ab; cd++) {
-        if (max_I[ab] * max_I[cd] * jb->P_max < cut->schwarz) continue;
-        int c = (int)((sqrt(8.0 * cd + 1.0) - 1.0) / 2.0);
-        while (c * (c + 1) / 2 > cd) c--;
-        while ((c + 1) * (c + 2) / 2 <= cd) c++;
-        int d = (int)(cd - c * (c + 1) / 2);
-        int atom_c = s->atom[c], atom_d = s->atom[d];
-        int cd_same = atom_c == atom_d;
-        /* Schwarz_test */
-        double JP = Pmax[ab] > Pmax[cd] ? Pmax[ab] : Pmax[cd];
-        double KP = Pmax[pidx(a, c)];
-        if (Pmax[pidx(a, d)] > KP) KP = Pmax[pidx(a, d)];
-        if (Pmax[pidx(b, c)] > KP) KP = Pmax[pidx(b, c)];
-        if (Pmax[pidx(b, d)] > KP) KP = Pmax[pidx(b, d)];
-        int skip_J = (JP < cut->jp) || (JP * max_I[ab] * max_I[cd] < cut->schwarz);
-        int skip_K = (KP < cut->kp) || (KP * max_I[ab] * max_I[cd] < cut->schwarz);
-        if (skip_J && skip_K) continue;
+        int c, d;
+        decode(cd, &c, &d);
+        const int code = screen_quartet(jb, ab, cd, a, b, c, d);
+        if (!code) continue;
+        const int skip_J = !(code & 1), skip_K = !(code & 2);
         double f = (a == b) ? 0.5 : 1.0;
         if (c == d) f *= 0.5;
         if (a == c && b == d) f *= 0.5;
-        int any_same = ab_same || cd_same || atom_a == atom_c || atom_a == atom_d ||
-                       atom_b == atom_c || atom_b == atom_d;
+        const int any_same = !uses_skip_lists(s, a, b, c, d);
         long npq = quartet(s, a, b, c, d, !any_same, cut, I, jb->ws[w]);
         st[0]++; st[1] += npq; st[2] += skip_J; st[3] += skip_K; st[4] += !any_same;
         int na = ncomp(s->l[a]), nb = ncomp(s->l[b]), nc = ncomp(s->l[c]), nd = ncomp(s->l[d]);
@@ -773,4 +809,81 @@ void orc_make_r_JK_sample(const osys_t *s, const ocut_t *cut, const double *P, c
                           double *J, double *K, long *stats, long stride)
 {
     make_r_JK_impl(s, cut, P, max_I_in, J, K, stats, stride < 1 ? 1 : stride, NULL, NULL, NULL);
+}
+
+/* ---------------- screening audit (tests only) ------------------------------------
+ * For every unique quartet (ab >= cd, q = ab(ab+1)/2 + cd, Tonto's pair order) code[q]
+ * is what the build above does with it (0: dropped; bit 0 J digested, bit 1 K digested,
+ * bit 2 skipped primitive-pair lists used) and maxabs[q] = max |(ab|cd)| over its
+ * components with full primitive lists, for every quartet, dropped or not: a test can
+ * then bound what each dropped quartet would have contributed from the integrals and the
+ * density themselves, without max_I or P_max. */
+typedef struct {
+    jk_job *jb;
+    unsigned char *code;
+    double *maxabs;
+    double **Ibuf;
+    pairdat_t **ws;
+} audit_job;
+
+static void audit_item(void *arg, long ab, int w)
+{
+    audit_job *aj = (audit_job *)arg;
+    jk_job *jb = aj->jb;
+    const osys_t *s = jb->s;
+    int a, b;
+    decode(ab, &a, &b);
+    const int row = ab_survives(jb, ab);
+    for (long cd = 0; cd <= ab; cd++) {
+        int c, d;
+        decode(cd, &c, &d);
+        const long q = ab * (ab + 1) / 2 + cd;
+        int code = row ? screen_quartet(jb, ab, cd, a, b, c, d) : 0;
+        if (code && uses_skip_lists(s, a, b, c, d)) code |= 4;
+        aj->code[q] = (unsigned char)code;
+        double *I = aj->Ibuf[w];
+        quartet(s, a, b, c, d, 0, jb->cut, I, aj->ws[w]);
+        const int nI = ncomp(s->l[a]) * ncomp(s->l[b]) * ncomp(s->l[c]) * ncomp(s->l[d]);
+        double m = 0.0;
+        for (int i = 0; i < nI; i++)
+            if (fabs(I[i]) > m) m = fabs(I[i]);
+        aj->maxabs[q] = m;
+    }
+}
+
+long orc_quartet_audit(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I,
+                       unsigned char *code, double *maxabs)
+{
+    comp_tables();
+    const long npair = (long)s->n_shell * (s->n_shell + 1) / 2;
+    double *Pmax = malloc(sizeof(double) * npair);
+    max_density(s, P, Pmax);
+    double P_max = 0.0, I_max = 0.0;
+    for (long i = 0; i < npair; i++) {
+        if (Pmax[i] > P_max) P_max = Pmax[i];
+        if (max_I[i] > I_max) I_max = max_I[i];
+    }
+    jk_job jb;
+    memset(&jb, 0, sizeof jb);
+    jb.s = s; jb.cut = cut; jb.P = P; jb.max_I = max_I; jb.Pmax = Pmax;
+    jb.P_max = P_max; jb.IP_max = I_max * P_max;
+    const int nthr = orc_max_threads();
+    audit_job aj = {&jb, code, maxabs, malloc(sizeof(double *) * nthr), malloc(sizeof(pairdat_t *) * nthr)};
+    for (int t = 0; t < nthr; t++) {
+        aj.Ibuf[t] = malloc(sizeof(double) * 10 * 10 * 10 * 10);
+        aj.ws[t] = malloc(2 * sizeof(pairdat_t));
+    }
+    parallel_for(npair, nthr, audit_item, &aj);
+    for (int t = 0; t < nthr; t++) { free(aj.Ibuf[t]); free(aj.ws[t]); }
+    free(aj.Ibuf); free(aj.ws); free(Pmax);
+    return npair * (npair + 1) / 2;
+}
+
+/* Number of primitive pairs of shell pair (a, b) with or without the 1e-6 skip
+ * (SHELL1QUARTET:set_ab_new, shell1quartet.foo:229-273). */
+int orc_pair_prims(const osys_t *s, int a, int b, int skip, double pair_cut)
+{
+    pairdat_t pd;
+    make_pair(s, a, b, skip, pair_cut, &pd);
+    return pd.n;
 }

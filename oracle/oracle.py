@@ -43,6 +43,11 @@ def lib():
         _lib.orc_set_threads.argtypes = [ctypes.c_int]
         _lib.orc_max_threads.restype = ctypes.c_int
         _lib.orc_component.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        _lib.orc_quartet_audit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, d, d, ctypes.c_void_p, d]
+        _lib.orc_quartet_audit.restype = ctypes.c_long
+        _lib.orc_pair_prims.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_double]
+        _lib.orc_pair_prims.restype = ctypes.c_int
+        _lib.orc_set_plant.argtypes = [ctypes.c_int]
     return _lib
 
 
@@ -100,6 +105,24 @@ class OracleSystem:
             return J, K, dict(quartets=st[0], prim_quartets=st[1], skip_J=st[2], skip_K=st[3],
                               skip_lists=st[4])
         return J, K
+
+    def quartet_audit(self, P, max_I, schwarz=1e-9, jp=1e-9, kp=1e-9, pair=1e-6):
+        """Per unique quartet (ab >= cd, index ab(ab+1)/2 + cd): what the build does with
+        it (0 dropped; bit 0 J, bit 1 K digested; bit 2 skipped primitive lists) and the
+        exact max |(ab|cd)| (full primitive lists)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        mi = np.ascontiguousarray(max_I, dtype=np.float64)
+        npair = self.n_pairs
+        nq = npair * (npair + 1) // 2
+        code = np.zeros(nq, dtype=np.uint8)
+        maxabs = np.zeros(nq)
+        cut = _OCut(schwarz, jp, kp, pair, 0)
+        lib().orc_quartet_audit(ctypes.byref(self.s), ctypes.byref(cut), _dp(P), _dp(mi),
+                                code.ctypes.data, _dp(maxabs))
+        return code, maxabs
+
+    def pair_prims(self, a, b, skip, pair=1e-6):
+        return lib().orc_pair_prims(ctypes.byref(self.s), a, b, int(skip), pair)
 
     def make_u_JK(self, Pa, Pb, schwarz=1e-9, jp=1e-9, kp=1e-9, pair=1e-6, max_I=None, return_stats=False):
         """UHF J/K as Tonto's make_u_JK_engine returns them (molecule.fock.foo:2558-2712):
@@ -181,6 +204,11 @@ def boys_quad(m, X):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def set_plant(k):
+    """Planted screening mistake (0: none); only the test of the screening audit sets it."""
+    lib().orc_set_plant(int(k))
 
 
 def max_threads():

@@ -77,3 +77,19 @@ def test_guess_independence(oracle_mod):
     r2 = run_rhf(torch.from_numpy(T + V), torch.from_numpy(S), mol.n_electrons,
                  mol.nuclear_repulsion(), jk, P0=P0, convergence=1e-10, diis_tol=1e-7)
     assert abs(r1.energy - r2.energy) < 1e-8
+
+
+@pytest.mark.parametrize("which,ref", [("nh3", -56.2093), ("urea", -221.0034)])
+def test_skip_path_molecules(oracle_mod, which, ref):
+    """More molecules with quartets over four distinct atoms (the 1e-6 primitive-pair skip
+    live): NH3/DZP, tests/short/nh3_rhf_DZP_HAF_and_structure_factors/stdout:628 (-56.2093,
+    V_NN 11.9664), and urea/STO-3G, tests/short/urea_rhf_STO-3G_HA_1e_properties/stdout:31
+    (-221.0034, V_NN 124.4944); both isolated molecules at the jobs' input geometries."""
+    from tonto_amd.geometry import tonto_nh3, tonto_urea
+    mol = tonto_nh3() if which == "nh3" else tonto_urea()
+    assert round(mol.nuclear_repulsion(), 4) == {"nh3": 11.9664, "urea": 124.4944}[which]
+    r = oracle_scf(oracle_mod, mol, conv=1e-9, tol=1e-6)
+    assert r.converged
+    assert round(r.energy, 4) == pytest.approx(ref, abs=1e-9)
+    _, _, st = oracle_mod.OracleSystem(mol).make_r_JK(r.P.numpy(), return_stats=True)
+    assert st["skip_lists"] > 0

@@ -91,3 +91,31 @@ def cn_hcn_dimer(basis_name="cc-pVDZ"):
         [[-2.703329, 0, 0], [-1.543137, 0, 0], [0.344846, 0, 0], [1.439363, 0, 0],
          [2.577273, 0, 0]],
         basis_name, charge=-1)
+
+
+def _from_fractional(symbols, frac, cell_ang, basis_name):
+    """Orthogonal cell: Cartesian Angstrom = fractional * cell edge."""
+    xyz = np.asarray(frac, dtype=np.float64) * np.asarray(cell_ang, dtype=np.float64)
+    return Molecule.from_angstrom(symbols, xyz, basis_name)
+
+
+def tonto_nh3(basis_name="DZP"):
+    """NH3 of `tests/short/nh3_rhf_DZP_HAF_and_structure_factors` (stdin: fractional
+    coordinates in a 5.1305 Angstrom cubic cell); its first SCF is the isolated molecule
+    (stdout:628-630, E = -56.2093, V_NN = 11.9664)."""
+    return _from_fractional(["N", "H", "H", "H"],
+                            [[0.2103, 0.2103, 0.2103], [0.3722, 0.2627, 0.1113],
+                             [0.1113, 0.3722, 0.2627], [0.2627, 0.1113, 0.3722]],
+                            [5.1305] * 3, basis_name)
+
+
+def tonto_urea(basis_name="STO-3G"):
+    """Urea of `tests/short/urea_rhf_STO-3G_HA_1e_properties` (stdin: fractional
+    coordinates, tetragonal cell 5.5780 x 5.5780 x 4.6860 Angstrom; stdout: E = -221.0034,
+    V_NN = 124.4944)."""
+    return _from_fractional(["O", "N", "N", "C", "H", "H", "H", "H"],
+                            [[0.0, 0.5, 0.596333], [0.144896, 0.644896, 0.178139],
+                             [-0.144896, 0.355104, 0.178139], [0.0, 0.5, 0.328189],
+                             [0.2557, 0.7557, 0.2841], [-0.2557, 0.2443, 0.2841],
+                             [0.1431, 0.6431, -0.0348], [-0.1431, 0.3569, -0.0348]],
+                            [5.5780, 5.5780, 4.6860], basis_name)
