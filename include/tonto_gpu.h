@@ -123,7 +123,9 @@ int tgpu_schwarz(tgpu_ctx *ctx, double *max_I);
 int tgpu_make_r_JK(tgpu_ctx *ctx, const double *P, double *J, double *K, int flags);
 
 /* Same on device pointers (n_bf*n_bf doubles each), asynchronous on the context's
- * stream.  No allocation; P must stay valid until the stream reaches this work. */
+ * stream; P must stay valid until the stream reaches this work.  Allocates nothing once
+ * tgpu_schwarz has run (a first call without it runs tgpu_schwarz, which allocates the
+ * wavefront-form workspace of f-shell bases), except the hipGraph of TGPU_JK_GRAPH. */
 int tgpu_make_r_JK_device(tgpu_ctx *ctx, const double *P, double *J, double *K, int flags);
 
 /* UHF J and K from the alpha and beta densities Pa, Pb, host arrays of n_bf*n_bf
@@ -132,7 +134,8 @@ int tgpu_make_r_JK_device(tgpu_ctx *ctx, const double *P, double *J, double *K, 
  * uses max(|Pa|, |Pb|) per shell pair (3670-3701), Ka = K(Pa), Kb = K(Pb), and
  * Ja + Jb = J(Pa + Pb) with Tonto's split between them (the engine adds the beta
  * density's ket-side J into Ja, JBcd => JA(cd), :2637).  flags: 0 or TGPU_JK_MIXED.
- * Synchronous; the statistics afterwards are those of the engine's quartets. */
+ * Synchronous; afterwards tgpu_get_stats gives the engine's quartet counters and the device
+ * time of all three passes. */
 int tgpu_make_u_JK(tgpu_ctx *ctx, const double *Pa, const double *Pb, double *Ja, double *Jb,
                    double *Ka, double *Kb, int flags);
 

@@ -187,6 +187,15 @@ typedef struct {
     int big_is_first;      /* Tonto: At = pos_a if la > lb else pos_b */
 } pairdat_t;
 
+/* per-worker scratch of one quartet evaluation: the two pairs, the (e0|f0) block and the
+ * transfer-relation buffers (sizes cover (ff|ff): 74 x 74 (e0|f0) entries, transfer levels
+ * up to 150 x 100) */
+#define QBUF 16384
+typedef struct {
+    pairdat_t pd[2];
+    double G[QBUF], escd[QBUF], abcd[QBUF], t0[QBUF], t1[QBUF];
+} qws_t;
+
 static void make_pair(const osys_t *s, int a, int b, int skip, double pair_cut, pairdat_t *pd)
 {
     const double *A = s->atom_xyz + 3 * s->atom[a], *B = s->atom_xyz + 3 * s->atom[b];
@@ -231,7 +240,7 @@ static void make_pair(const osys_t *s, int a, int b, int skip, double pair_cut, 
  * out: [lead][ncomp(lbig)][ncomp(lsm)][trail]
  * (a, b+1_t) = (a+1_t, b) + R_t (a, b), R = big - small centre. */
 static void hrr(int lbig, int lsm, const double *R, const double *in, int lead, int trail,
-                double *out)
+                double *out, double *t0, double *t1)
 {
     int base = ncomp_up_to(lbig - 1);
     int nin = ncomp_up_to(lbig + lsm) - base;
@@ -240,12 +249,12 @@ static void hrr(int lbig, int lsm, const double *R, const double *in, int lead, 
         return;
     }
     /* level k: a in l range [lbig, lbig+lsm-k] (na_k comps), b comps of l = k */
-    double *cur = malloc(sizeof(double) * lead * nin * trail);
-    memcpy(cur, in, sizeof(double) * lead * nin * trail);
+    const double *cur = in;
     int na = nin, nb = 1;
     for (int k = 0; k < lsm; k++) {
         int na1 = ncomp_up_to(lbig + lsm - k - 1) - base, nb1 = ncomp(k + 1);
-        double *nxt = malloc(sizeof(double) * lead * na1 * nb1 * trail);
+        double *nxt = k == lsm - 1 ? out : (k % 2 ? t1 : t0);
+        if ((long)lead * na1 * nb1 * trail > QBUF) abort();
         int gb0 = ncomp_up_to(k - 1), gb1 = ncomp_up_to(k);
         for (int ib1 = 0; ib1 < nb1; ib1++) {
             int g1 = gb1 + ib1, t = 0;
@@ -262,21 +271,18 @@ static void hrr(int lbig, int lsm, const double *R, const double *in, int lead, 
                             R[t] * cur[((L * na + ia) * nb + ib) * trail + T];
                 }
         }
-        free(cur);
         cur = nxt; na = na1; nb = nb1;
     }
-    /* na == ncomp(lbig) */
-    memcpy(out, cur, sizeof(double) * lead * na * nb * trail);
-    free(cur);
+    /* the last level wrote out: na == ncomp(lbig) */
 }
 
 /* ---------------- one shell quartet (ab|cd) ------------------------------------- */
 /* out[((ia*nb + ib)*nc + ic)*nd + id]; returns number of primitive quartets */
 static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const ocut_t *cut,
-                    double *out, pairdat_t *ws)
+                    double *out, qws_t *ws)
 {
     comp_tables();
-    pairdat_t *pab = &ws[0], *pcd = &ws[1];
+    pairdat_t *pab = &ws->pd[0], *pcd = &ws->pd[1];
     make_pair(s, a, b, skip, cut->pair, pab);
     make_pair(s, c, d, skip, cut->pair, pcd);
     int la = s->l[a], lb = s->l[b], lc = s->l[c], ld = s->l[d];
@@ -286,7 +292,9 @@ static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const
     int e0 = ncomp_up_to(lbig_ab - 1), ne = ncomp_up_to(lab) - e0;
     int f0 = ncomp_up_to(lbig_cd - 1), nf = ncomp_up_to(lcd) - f0;
     int nroot = (lab + lcd) / 2 + 1;
-    double *G = calloc((size_t)ne * nf, sizeof(double));
+    if (ne * nf > QBUF) abort();
+    double *G = ws->G;
+    memset(G, 0, sizeof(double) * ne * nf);
     double Ix[9][9], Iy[9][9], Iz[9][9];
     double u[16], w[16];
     for (int p = 0; p < pab->n; p++) {
@@ -332,11 +340,11 @@ static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const
     /* HRR on cd (trailing index), then ab */
     int nc = ncomp(lc), nd = ncomp(ld), na_ = ncomp(la), nb_ = ncomp(lb);
     int nbig_cd = ncomp(lbig_cd), nsm_cd = ncomp(lsm_cd);
-    double *escd = malloc(sizeof(double) * ne * nbig_cd * nsm_cd);
-    hrr(lbig_cd, lsm_cd, pcd->R, G, ne, 1, escd);            /* [e][big][small] */
-    int nbig_ab = ncomp(lbig_ab), nsm_ab = ncomp(lsm_ab);
-    double *abcd = malloc(sizeof(double) * nbig_ab * nsm_ab * nbig_cd * nsm_cd);
-    hrr(lbig_ab, lsm_ab, pab->R, escd, 1, nbig_cd * nsm_cd, abcd); /* [big][small][cdbig][cdsmall] */
+    double *escd = ws->escd;
+    hrr(lbig_cd, lsm_cd, pcd->R, G, ne, 1, escd, ws->t0, ws->t1);   /* [e][big][small] */
+    int nsm_ab = ncomp(lsm_ab);
+    double *abcd = ws->abcd;
+    hrr(lbig_ab, lsm_ab, pab->R, escd, 1, nbig_cd * nsm_cd, abcd, ws->t0, ws->t1); /* [big][small][cdbig][cdsmall] */
     for (int ia = 0; ia < na_; ia++)
         for (int ib = 0; ib < nb_; ib++)
             for (int ic = 0; ic < nc; ic++)
@@ -346,7 +354,6 @@ static long quartet(const osys_t *s, int a, int b, int c, int d, int skip, const
                     out[((ia * nb_ + ib) * nc + ic) * nd + id] =
                         abcd[((i1 * nsm_ab + i2) * nbig_cd + i3) * nsm_cd + i4];
                 }
-    free(abcd); free(escd); free(G);
     return nprimq;
 }
 
@@ -355,7 +362,7 @@ long orc_eri_quartet(const osys_t *s, int a, int b, int c, int d, int skip, doub
                      int rys_quad, double *out)
 {
     ocut_t cut = {0, 0, 0, pair_cut, rys_quad};
-    pairdat_t *ws = malloc(2 * sizeof(pairdat_t));
+    qws_t *ws = malloc(sizeof(qws_t));
     long n = quartet(s, a, b, c, d, skip, &cut, out, ws);
     free(ws);
     return n;
@@ -372,7 +379,7 @@ static void schwarz_item(void *arg, long item, int worker)
     int a = (int)item;
     {
         double *buf = malloc(sizeof(double) * 10 * 10 * 10 * 10);
-        pairdat_t *ws = malloc(2 * sizeof(pairdat_t));
+        qws_t *ws = malloc(sizeof(qws_t));
         for (int b = 0; b <= a; b++) {
             quartet(s, a, b, a, b, 0, &jb->cut, buf, ws);
             int na = ncomp(s->l[a]), nb = ncomp(s->l[b]);
@@ -440,7 +447,7 @@ typedef struct {
     double **Jbt, **Kbt;          /* UHF: per-worker B-spin J and K */
     double P_max, IP_max;
     double **Jt, **Kt, **Ibuf;
-    pairdat_t **ws;
+    qws_t **ws;
     long (*st)[5];
     long sample_stride;
 } jk_job;
@@ -600,7 +607,7 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
     jb.Jbt = malloc(sizeof(double *) * nthr);
     jb.Kbt = malloc(sizeof(double *) * nthr);
     jb.Ibuf = malloc(sizeof(double *) * nthr);
-    jb.ws = malloc(sizeof(pairdat_t *) * nthr);
+    jb.ws = malloc(sizeof(qws_t *) * nthr);
     jb.st = calloc(nthr, sizeof(long[5]));
     for (int t = 0; t < nthr; t++) {
         jb.Jt[t] = calloc((size_t)n * n, sizeof(double));
@@ -608,7 +615,7 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
         jb.Jbt[t] = Pb ? calloc((size_t)n * n, sizeof(double)) : NULL;
         jb.Kbt[t] = Pb ? calloc((size_t)n * n, sizeof(double)) : NULL;
         jb.Ibuf[t] = malloc(sizeof(double) * 10 * 10 * 10 * 10);
-        jb.ws[t] = malloc(2 * sizeof(pairdat_t));
+        jb.ws[t] = malloc(sizeof(qws_t));
     }
     if (sample_stride > 1) {
         /* bounded sample of the same build: every sample_stride-th (ab| pair */
@@ -823,7 +830,7 @@ typedef struct {
     unsigned char *code;
     double *maxabs;
     double **Ibuf;
-    pairdat_t **ws;
+    qws_t **ws;
 } audit_job;
 
 static void audit_item(void *arg, long ab, int w)
@@ -868,10 +875,10 @@ long orc_quartet_audit(const osys_t *s, const ocut_t *cut, const double *P, cons
     jb.s = s; jb.cut = cut; jb.P = P; jb.max_I = max_I; jb.Pmax = Pmax;
     jb.P_max = P_max; jb.IP_max = I_max * P_max;
     const int nthr = orc_max_threads();
-    audit_job aj = {&jb, code, maxabs, malloc(sizeof(double *) * nthr), malloc(sizeof(pairdat_t *) * nthr)};
+    audit_job aj = {&jb, code, maxabs, malloc(sizeof(double *) * nthr), malloc(sizeof(qws_t *) * nthr)};
     for (int t = 0; t < nthr; t++) {
         aj.Ibuf[t] = malloc(sizeof(double) * 10 * 10 * 10 * 10);
-        aj.ws[t] = malloc(2 * sizeof(pairdat_t));
+        aj.ws[t] = malloc(sizeof(qws_t));
     }
     parallel_for(npair, nthr, audit_item, &aj);
     for (int t = 0; t < nthr; t++) { free(aj.Ibuf[t]); free(aj.ws[t]); }

@@ -344,3 +344,24 @@ def test_j_only_and_k_only_builds(torch_cuda, oracle_mod, only):
         assert not K.any() and np.max(np.abs(J - Jo)) <= 1e-10
     else:
         assert not J.any() and np.max(np.abs(K - Ko)) <= 1e-10
+
+
+def test_uhf_mixed_precision(torch_cuda, oracle_mod):
+    """The UHF build with TGPU_JK_MIXED on (H2O)4 / 6-31G(d) at 1e-12: every pass runs its
+    small-bound quartets in FP32; the Fock-relevant combinations Ja + Jb - Ka and
+    Ja + Jb - Kb stay within 1e-7 of the FP64 UHF build and the quartet set is the same."""
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(4, "6-31G(d)", seed=5)
+    e = _engine(mol, schwarz=1e-12, fp32_threshold=1e-6)
+    e.schwarz()
+    Pa = _rand_density(mol.n_bf, 23, 0.5)
+    Pb = _rand_density(mol.n_bf, 24, 0.4)
+    Ja, Jb, Ka, Kb = e.make_u_JK(Pa, Pb)
+    q64 = e.stats()
+    Jam, Jbm, Kam, Kbm = e.make_u_JK(Pa, Pb, mixed=True)
+    qmx = e.stats()
+    assert qmx["fp32_quartets"] > 0 and qmx["quartets"] == q64["quartets"]
+    dFa = np.max(np.abs((Jam + Jbm - Kam) - (Ja + Jb - Ka)))
+    dFb = np.max(np.abs((Jam + Jbm - Kbm) - (Ja + Jb - Kb)))
+    print("UHF mixed dF", dFa, dFb, qmx)
+    assert dFa < 1e-7 and dFb < 1e-7

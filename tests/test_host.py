@@ -64,3 +64,25 @@ def test_scf_driver_on_a_model_problem():
     F = h + 0.3 * r.P - 0.1 * r.P
     e, C = torch.linalg.eigh(F)
     assert r.converged and torch.allclose(r.P, 2 * C[:, :1] @ C[:, :1].T, atol=1e-9)
+
+
+def test_sixteen_bit_packing_round_trip(tmp_path):
+    """Pair records and ket queues pack two 16-bit fields (first basis functions up to
+    65535) into one int: the upper field must survive the round trip above 32767, where
+    it sets the sign bit.  Host build of the device header's pack16/hi16/lo16."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "pack.cpp"
+    src.write_text(
+        '#include "tgpu_device.h"\n#include <cstdio>\n'
+        "int main() { int bad = 0;\n"
+        "  const int v[] = {0, 1, 32767, 32768, 40000, 65535};\n"
+        "  for (int hi : v) for (int lo : v) { const int p = pack16(hi, lo);\n"
+        "    if (hi16(p) != hi || lo16(p) != lo) ++bad; }\n"
+        "  std::printf(\"%d\\n\", bad); return bad != 0; }\n")
+    exe = tmp_path / "pack"
+    subprocess.check_call(["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-std=c++17",
+                           "-I", os.path.join(root, "tonto_amd", "csrc"), str(src), "-o", str(exe)])
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "0"

@@ -70,3 +70,26 @@ def test_rebuild_every_counts_builds_since_the_last_full_one(oracle_mod):
         run = 0 if m == "full" else run + 1
         assert run <= 2
     assert np.isfinite(r.energy)
+
+
+def test_staged_schedule_switches_on_the_diis_error(oracle_mod):
+    """switch_error: the low-precision phase (loose full builds, then delta builds) runs
+    until the DIIS error of its own Fock matrix is below the threshold; then one full FP64
+    build and FP64 delta builds to convergence, with no extra final build (final_full
+    False).  Energy within 1e-8 of the plain FP64 run."""
+    mol, h, S, jk, calls = _system(oracle_mod)
+    ref = run_rhf(h, S, mol.n_electrons, mol.nuclear_repulsion(), jk, convergence=1e-9, diis_tol=1e-6)
+    calls.clear()
+    r = run_rhf(h, S, mol.n_electrons, mol.nuclear_repulsion(), jk, convergence=1e-9, diis_tol=1e-6,
+                mixed_until=1e-5, incremental=True, rebuild_every=100, loose_scale=100.0,
+                switch_error=1e-4, final_full=False)
+    assert r.converged and abs(r.energy - ref.energy) <= 1e-8
+    modes = [m for m, _ in calls]
+    errs = [h_["diis_error"] for h_ in r.history]
+    first64 = next(i for i, m in enumerate(modes) if not m.endswith("_mixed"))
+    # the switch comes right after the first iteration whose low-precision DIIS error < 1e-4
+    k_switch = next(k for k, e in enumerate(errs) if e < 1e-4)
+    assert first64 == k_switch + 2 and modes[first64] == "full"
+    assert all(m == "delta" for m in modes[first64 + 1:])
+    assert "delta_mixed" in modes[:first64]
+    assert len(modes) == r.iterations + 1

@@ -17,8 +17,8 @@
 #define TGPU_SCHWARZ_NAME_X(a, b) TGPU_CAT3(tgpu_launch_schwarz_, a, b)
 #define TGPU_WS_NAME_X(a, b, c, d) TGPU_CAT5(tgpu_wave_ws_, a, b, c, d)
 
-void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int variant, bool mixed,
-                                                          hipStream_t s)
+int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int variant, bool mixed,
+                                                         hipStream_t s)
 {
 #if !TGPU_HEAVY
     if (variant == 0) {
@@ -52,7 +52,7 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
             const size_t rows = (size_t)nkr * row_bytes;
             if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(b);
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(b);
-            return;
+            return 0;
         }
 #else
         (void)NP;
@@ -61,7 +61,7 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
         const int blocks = (a.n_bra + 3) / 4;
         if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, false><<<blocks, 256, 0, s>>>(a);
         else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, false><<<blocks, 256, 0, s>>>(a);
-        return;
+        return 0;
     }
 #endif
     (void)variant;
@@ -71,13 +71,14 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
     // a.ket_chunk kets.  gy splits long prefixes when there are few bra pairs, so that the
     // grid holds about kWaveGrid work-groups; workspace classes stay within their slots.
     constexpr long long kWaveGrid = 65536;
-    if (a.n_bra <= 0) return;
+    if (a.n_bra <= 0) return 0;
     const int kc = a.ket_chunk > 0 && a.ket_chunk <= 64 ? a.ket_chunk : 64;  // one lane per ket
     const long long nch = a.n_ket_max > 0 ? (a.n_ket_max + kc - 1) / kc : 1;
     long long gx = a.n_bra, gy;
     if (C::GB) {
         const long long slots = ws_slot_count(a.ws_doubles, C::WS);
-        if (slots <= 0 || !a.ws) return;  // unreachable: tgpu_schwarz allocates it first
+        // the caller sizes the workspace for at least one slot of every class it launches
+        if (slots <= 0 || !a.ws) return TGPU_E_STATE;
         gx = std::min<long long>(gx, slots);
         gy = std::min<long long>(nch, std::max<long long>(1, slots / gx));
     } else {
@@ -87,6 +88,7 @@ void TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int
     JKArgs b = a;
     b.ket_chunk = kc;
     jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3((unsigned)gx, (unsigned)gy), 64, 0, s>>>(b);
+    return 0;
 }
 
 size_t TGPU_WS_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)()

@@ -11,6 +11,12 @@
 
 #define TWOPI5ON2 34.9868366552497256925  // include/macros.in:446
 
+// Rys coefficient loads of the lane-per-quartet kernels: batches of about this many
+// (root, weight) pairs issued together (0: one step of the recurrence at a time)
+#ifndef TGPU_RYS_BATCH
+#define TGPU_RYS_BATCH 0
+#endif
+
 // ----------------------------------------------------------------------------------
 // Device helpers
 // ----------------------------------------------------------------------------------
@@ -149,6 +155,34 @@ __device__ __forceinline__ void rys_roots_tab(double X, double *u, double *w, co
         b1[f] = 0.0;
         b2[f] = 0.0;
     }
+#if TGPU_RYS_BATCH
+    // coefficients m = 13 .. 1 in batches of G, each batch's loads issued before the
+    // recurrence uses any of them, so that their latencies overlap (2N*G doubles in
+    // flight)
+    constexpr int G = N == 1 ? TGPU_RYS_BATCH : (TGPU_RYS_BATCH + N - 1) / N;
+#pragma unroll
+    for (int m0 = RYS_NCHEB - 1; m0 >= 1; m0 -= G) {
+        double2 cb[G][N];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int h = 0; h < N; ++h)
+                if (m0 - g >= 1) cb[g][h] = c[(m0 - g) * N + h];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (m0 - g < 1) break;
+#pragma unroll
+            for (int h = 0; h < N; ++h) {
+                const double b0x = tt * b1[2 * h] - b2[2 * h] + cb[g][h].x;
+                const double b0y = tt * b1[2 * h + 1] - b2[2 * h + 1] + cb[g][h].y;
+                b2[2 * h] = b1[2 * h];
+                b1[2 * h] = b0x;
+                b2[2 * h + 1] = b1[2 * h + 1];
+                b1[2 * h + 1] = b0y;
+            }
+        }
+    }
+#else
 #pragma unroll 2
     for (int m = RYS_NCHEB - 1; m >= 1; --m) {
 #pragma unroll
@@ -162,6 +196,7 @@ __device__ __forceinline__ void rys_roots_tab(double X, double *u, double *w, co
             b1[2 * h + 1] = b0y;
         }
     }
+#endif
 #pragma unroll
     for (int h = 0; h < N; ++h) {
         const double2 c0 = c[h];
@@ -207,8 +242,11 @@ struct DevPairs {
     const int4 *rec0, *rec1;
     const double *A, *AB, *maxI;
 };
-__device__ __forceinline__ int hi16(int v) { return (int)((unsigned)v >> 16); }
-__device__ __forceinline__ int lo16(int v) { return v & 0xffff; }
+// two 16-bit fields (0..65535) in one int; the upper one may set the sign bit, so it is
+// packed and unpacked unsigned
+__host__ __device__ __forceinline__ int pack16(int hi, int lo) { return (int)(((unsigned)hi << 16) | ((unsigned)lo & 0xffffu)); }
+__host__ __device__ __forceinline__ int hi16(int v) { return (int)((unsigned)v >> 16); }
+__host__ __device__ __forceinline__ int lo16(int v) { return v & 0xffff; }
 
 __device__ __forceinline__ PairView make_view(const DevPairs &pr, const DevPrims &pm, int s,
                                               bool skip)
@@ -592,7 +630,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                 q[pos] = make_int2((j << 5) | (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) |
                                        (t.halvings << 3),
-                                   (t.fr << 16) | t.fs);
+                                   pack16(t.fr, t.fs));
             }
             qn += __popcll(m);
             wave_lds_sync();
@@ -604,7 +642,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
             const int2 e = lane < take ? q[base + lane] : make_int2(0, 0);
             qn = base;
             wave_lds_sync();
-            if (lane < take) quartet(e.x >> 5, e.x & 31, e.y >> 16, e.y & 0xffff,
+            if (lane < take) quartet(e.x >> 5, e.x & 31, hi16(e.y), lo16(e.y),
                                      make_view(a.pr, a.pm, b.sb, e.x & 4));
         }
         if (drain && qn == 0) break;
@@ -885,7 +923,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         const int flags_l = ok ? ((int)tl.skipJ | ((int)tl.skipK << 1) | ((int)tl.skip_lists << 2) |
                                   (tl.halvings << 3))
                                : 0;
-        const int frs_l = ok ? (tl.fr << 16) | tl.fs : 0;
+        const int frs_l = ok ? pack16(tl.fr, tl.fs) : 0;
         unsigned long long pass = __ballot(ok);
     while (pass) {  // uniform over the work-group
         const int l = __builtin_ctzll(pass);
@@ -901,7 +939,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
         const double f = t.f;
-        const int fr = frs >> 16, fs = frs & 0xffff;
+        const int fr = hi16(frs), fs = lo16(frs);
         // stage the ket-dependent density blocks (P symmetric: the K-side blocks come from
         // the bra pair's rows); when each fits one pass of the 64 lanes the five loads are
         // issued together and waited for once
@@ -1048,7 +1086,7 @@ __global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims 
 // Launchers defined in the per-class translation units (jk_class.hip).  variant 0: one
 // lane per quartet (light classes only); variant 1: one wavefront per quartet.
 #define TGPU_DECLARE_JK(LP, LQ, LR, LS, H) \
-    void tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, int variant, bool mixed, hipStream_t s); \
+    int tgpu_launch_jk_##LP##LQ##LR##LS(const JKArgs &a, int variant, bool mixed, hipStream_t s); \
     size_t tgpu_wave_ws_##LP##LQ##LR##LS();
 TGPU_FOR_EACH_CLASS(TGPU_DECLARE_JK)
 #undef TGPU_DECLARE_JK

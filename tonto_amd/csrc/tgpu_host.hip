@@ -450,12 +450,12 @@ static void pack_records(const tgpu_ctx *c, std::vector<int> &r0, std::vector<in
     r1.resize(4 * np);
     for (size_t s = 0; s < np; ++s) {
         r0[4 * s + 0] = c->tonto[s];
-        r0[4 * s + 1] = (c->sp[s] << 16) | c->sq[s];
-        r0[4 * s + 2] = (c->atp[s] << 16) | c->atq[s];
-        r0[4 * s + 3] = (c->bfp[s] << 16) | c->bfq[s];
+        r0[4 * s + 1] = pack16(c->sp[s], c->sq[s]);
+        r0[4 * s + 2] = pack16(c->atp[s], c->atq[s]);
+        r0[4 * s + 3] = pack16(c->bfp[s], c->bfq[s]);
         r1[4 * s + 0] = c->off_full[s];
         r1[4 * s + 1] = c->off_skip[s];
-        r1[4 * s + 2] = (c->n_full[s] << 16) | c->n_skip[s];
+        r1[4 * s + 2] = pack16(c->n_full[s], c->n_skip[s]);
         r1[4 * s + 3] = c->pmol[s];
     }
 }
@@ -779,15 +779,10 @@ static const int kClassPair[NQCLS][2] = {
 // Allocates the wavefront-form workspace for the class pairs this basis has (none for
 // s/p/d bases): room for at least 1024 slots of the largest per-slot need, and up to
 // 8192 slots or 1 GiB, so classes with smaller transfer buffers run wider grids.
+static size_t workspace_need(const tgpu_ctx *c);
 static int ensure_workspace(tgpu_ctx *c)
 {
-    size_t need = 0;
-    for (int k = 0; k < NQCLS; ++k) {
-        const int CX = kClassPair[k][0], CY = kClassPair[k][1];
-        const int nb = c->bin_start[CX * NBK + NBK] - c->bin_start[CX * NBK];
-        const int nk = c->bin_start[CY * NBK + NBK] - c->bin_start[CY * NBK];
-        if (nb && nk) need = std::max(need, kWaveWs[k]());
-    }
+    const size_t need = workspace_need(c);
     if (!need) return 0;
     const size_t n = std::max(1024 * need, std::min(8192 * need, (size_t)(1 << 30) / sizeof(double)));
     if (int rc = dalloc(c, c->d_ws, n)) return rc;
@@ -844,7 +839,7 @@ extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
     return TGPU_OK;
 }
 
-typedef void (*jk_launcher)(const JKArgs &, int, bool, hipStream_t);
+typedef int (*jk_launcher)(const JKArgs &, int, bool, hipStream_t);
 static const jk_launcher kLaunch[NQCLS] = {
 #define TGPU_LAUNCHER(LP, LQ, LR, LS, H) tgpu_launch_jk_##LP##LQ##LR##LS,
     TGPU_FOR_EACH_CLASS(TGPU_LAUNCHER)
@@ -852,27 +847,58 @@ static const jk_launcher kLaunch[NQCLS] = {
 };
 static int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
 
-// Enqueue one complete build on stream s (no host synchronisation).  UHF passes: P2 is the
-// other spin's density (screening by max(|P|, |P2|)), j_ket_nonsame selects the J of
-// Tonto's ket side only (and no K).
-static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, int flags,
-                          hipStream_t s, bool timing, const double *P2 = nullptr, int j_ket_nonsame = 0)
+// Device buffers one build writes besides J and K: the accumulators X, Y (n_elem doubles
+// each), P_max per Tonto pair, the per-molecule maxima (2 per molecule), the class counters
+// (3 per class pair) and the wavefront-form workspace.
+struct BuildBufs {
+    double *X, *Y, *Pmax, *scal;
+    unsigned long long *counters;
+    double *ws;
+    long long ws_doubles;
+};
+
+static BuildBufs ctx_bufs(const tgpu_ctx *c)
+{
+    return BuildBufs{c->d_X.p, c->d_Y.p, c->d_Pmax.p, c->d_scal.p, c->d_counters.p, c->d_ws.p, c->ws_doubles};
+}
+
+// Largest per-slot workspace (doubles) of the wavefront-form class pairs this basis has
+// (0 for s/p/d bases).
+static size_t workspace_need(const tgpu_ctx *c)
+{
+    size_t need = 0;
+    for (int k = 0; k < NQCLS; ++k) {
+        const int CX = kClassPair[k][0], CY = kClassPair[k][1];
+        const int nb = c->bin_start[CX * NBK + NBK] - c->bin_start[CX * NBK];
+        const int nk = c->bin_start[CY * NBK + NBK] - c->bin_start[CY * NBK];
+        if (nb && nk) need = std::max(need, kWaveWs[k]());
+    }
+    return need;
+}
+
+// Enqueue one complete build on stream s (no host synchronisation) into the buffers B.
+// UHF passes: P2 is the other spin's density (screening by max(|P|, |P2|)), j_ket_nonsame
+// selects the J of Tonto's ket side only (and no K).  rec_start / rec_end record the
+// context's timing events around the class launches.
+static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, double *J, double *K, int flags,
+                         hipStream_t s, bool rec_start, bool rec_end, const double *P2 = nullptr,
+                         int j_ket_nonsame = 0)
 {
     const bool mixed = flags & TGPU_JK_MIXED;
     const BatchArgs ba = batch_args(c);
-    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_X.p, c->n_elem);
-    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(c->d_Y.p, c->n_elem);
-    fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(c->d_counters.p, 3 * NQCLS);
-    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(P, P2, ba, c->d_Pmax.p);
-    mol_max_kernel<<<c->n_mol, 256, 0, s>>>(c->d_Pmax.p, ba, c->d_maxI_mol.p, c->d_scal.p);
-    if (timing) (void)hipEventRecord(c->ev0, s);
+    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.X, c->n_elem);
+    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.Y, c->n_elem);
+    fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(B.counters, 3 * NQCLS);
+    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(P, P2, ba, B.Pmax);
+    mol_max_kernel<<<c->n_mol, 256, 0, s>>>(B.Pmax, ba, c->d_maxI_mol.p, B.scal);
+    if (rec_start) (void)hipEventRecord(c->ev0, s);
     JKArgs a;
     a.pr = dev_pairs(c);
     a.pm = dev_prims(c);
     a.ml = dev_mols(c);
     a.P = P;
-    a.Pmax = c->d_Pmax.p;
-    a.scal = c->d_scal.p;
+    a.Pmax = B.Pmax;
+    a.scal = B.scal;
     a.cutoff = c->cut_schwarz;
     a.jp = c->cut_jp;
     a.kp = c->cut_kp;
@@ -883,18 +909,18 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
     a.j_ket_nonsame = j_ket_nonsame;
     if (j_ket_nonsame) a.kp = HUGE_VAL;  // the pass digests no K
     a.fp32_thr = c->fp32_thr;
-    a.X = c->d_X.p;
-    a.Y = c->d_Y.p;
-    a.counters = c->d_counters.p;
-    a.ws = c->d_ws.p;
-    a.ws_doubles = c->ws_doubles;
+    a.X = B.X;
+    a.Y = B.Y;
+    a.counters = B.counters;
+    a.ws = B.ws;
+    a.ws_doubles = B.ws_doubles;
     a.max_nbf = *std::max_element(c->nbf.begin(), c->nbf.end());
     a.krow_rows = 0;
     a.krow_budget = c->krow_budget;
     a.krow_min_kets = c->krow_min_kets;
+    const bool prof = rec_end && c->profiling;
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];
-        const bool prof = timing && c->profiling;
         if (prof) c->cls_launched[k] = 0;
         const int nb = c->bin_start[CX * NBK + NBK] - c->bin_start[CX * NBK];
         const int nk = c->bin_start[CY * NBK + NBK] - c->bin_start[CY * NBK];
@@ -921,16 +947,19 @@ static void enqueue_build(tgpu_ctx *c, const double *P, double *J, double *K, in
                        (long long)a.n_bra * ((a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk) < 131072)
                     a.ket_chunk /= 2;
                 if (a.n_bra == 0 || c->bin_start[Y + 1] == c->bin_start[Y]) continue;
-                kLaunch[k](a, c->variant[k], mixed, s);
+                if (kLaunch[k](a, c->variant[k], mixed, s)) {
+                    c->err = "class pair " + std::to_string(k) + ": no wavefront-form workspace";
+                    return TGPU_E_STATE;
+                }
             }
         if (prof) {
             (void)hipEventRecord(c->cls_ev[2 * k + 1], s);
             c->cls_launched[k] = 1;
         }
     }
-    if (timing) (void)hipEventRecord(c->ev1, s);
-    post_kernel<<<dim3(grid_for(c->n_elem / c->n_mol + 1), c->n_mol), 256, 0, s>>>(c->d_X.p, c->d_Y.p,
-                                                                                   ba, J, K);
+    if (rec_end) (void)hipEventRecord(c->ev1, s);
+    post_kernel<<<dim3(grid_for(c->n_elem / c->n_mol + 1), c->n_mol), 256, 0, s>>>(B.X, B.Y, ba, J, K);
+    return TGPU_OK;
 }
 
 extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, double *K, int flags)
@@ -946,7 +975,7 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
     }
     const int kflags = flags & (TGPU_JK_MIXED | TGPU_J_ONLY | TGPU_K_ONLY);
     if (!(flags & TGPU_JK_GRAPH)) {
-        enqueue_build(c, P, J, K, kflags, c->stream, true);
+        if (int rc = enqueue_build(c, ctx_bufs(c), P, J, K, kflags, c->stream, true, true)) return rc;
         c->timed = true;
         HIPCHK(c, hipGetLastError());
         return TGPU_OK;
@@ -958,8 +987,12 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
         drop_graph(c);
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
-        enqueue_build(c, P, J, K, kflags, c->gstream, false);
+        const int brc = enqueue_build(c, ctx_bufs(c), P, J, K, kflags, c->gstream, false, false);
         HIPCHK(c, hipStreamEndCapture(c->gstream, &g));
+        if (brc) {
+            (void)hipGraphDestroy(g);
+            return brc;
+        }
         hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
@@ -1009,9 +1042,12 @@ extern "C" int tgpu_make_u_JK_device(tgpu_ctx *c, const double *Pa, const double
     const size_t nn = (size_t)c->n_elem;
     int rc;
     if ((rc = dalloc(c, c->d_uJ, nn)) || (rc = dalloc(c, c->d_uK, nn))) return rc;
-    enqueue_build(c, Pb, c->d_uJ.p, c->d_uK.p, flags, c->stream, false, Pa, 1);
-    enqueue_build(c, Pb, Jb, Kb, flags, c->stream, false, Pa, 0);
-    enqueue_build(c, Pa, Ja, Ka, flags, c->stream, true, Pb, 0);
+    // the timing events bracket all three passes; the class counters are the Pa pass's
+    const BuildBufs B = ctx_bufs(c);
+    if ((rc = enqueue_build(c, B, Pb, c->d_uJ.p, c->d_uK.p, flags, c->stream, true, false, Pa, 1)) ||
+        (rc = enqueue_build(c, B, Pb, Jb, Kb, flags, c->stream, false, false, Pa, 0)) ||
+        (rc = enqueue_build(c, B, Pa, Ja, Ka, flags, c->stream, false, true, Pb, 0)))
+        return rc;
     uhf_combine_kernel<<<grid_for(nn), 256, 0, c->stream>>>(Ja, Jb, c->d_uJ.p, nn);
     c->timed = true;
     HIPCHK(c, hipGetLastError());
@@ -1147,8 +1183,11 @@ extern "C" int tgpu_autotune(tgpu_ctx *c, const double *P, int flags, double *ms
     for (int v = 0; v < 2; ++v) {
         for (int k = 0; k < NQCLS; ++k) c->variant[k] = kHeavy[k] ? 1 : v;
         // one untimed pass so both forms start from warm caches
-        enqueue_build(c, P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream, false);
-        enqueue_build(c, P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream, true);
+        if ((rc = enqueue_build(c, ctx_bufs(c), P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream,
+                                false, false)) ||
+            (rc = enqueue_build(c, ctx_bufs(c), P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream,
+                                true, true)))
+            return rc;
         c->timed = true;
         HIPCHK(c, hipGetLastError());
         if ((rc = tgpu_get_class_times(c, t[v], NQCLS))) return rc;

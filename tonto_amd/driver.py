@@ -42,6 +42,10 @@ class GPUSCF:
         self.mixed_delta_density_cutoff = 1e-12
         # Schwarz cutoff of the mixed run's first full builds (see run)
         self.loose_cutoff = 1e-8
+        # mixed run: DIIS error at which the low-precision phase hands over to FP64, and
+        # whether a converged run makes one more full FP64 build (see run)
+        self.switch_error = 1e-4
+        self.final_full = False
 
     def autotune(self, P, mixed=False):
         """Pick the faster kernel form per class pair on this molecule and density."""
@@ -146,15 +150,29 @@ class GPUSCF:
         finally:
             self.engine.set_cutoffs_scaled(1.0)
 
-    def run(self, precision="fp64", mixed_until=1e-4, incremental=False, guess="promolecule",
+    def run(self, precision="fp64", mixed_until=1e-5, incremental=None, guess="promolecule",
             log=None, guess_model=None, **kw):
-        """RHF to convergence; the wall time includes making the initial guess.  A mixed
-        run makes its full builds at DIIS error >= 1e-2 (the first one included) with the
-        Schwarz cutoff loose_cutoff (1e-8; ~1e-5 in F, 2.2x faster builds on
-        (H2O)64 / cc-pVTZ) unless loose_scale is given."""
-        if precision == "mixed" and "loose_scale" not in kw:
-            s = self.engine.cutoffs[0]
-            kw["loose_scale"] = max(1.0, self.loose_cutoff / s) if s > 0 else 1.0
+        """RHF to convergence; the wall time includes making the initial guess.
+
+        precision="fp64": a full FP64 build at the engine's cutoffs every iteration (the
+        plain path).  precision="mixed" (defaults, each overridable through kw):
+          * builds made while the DIIS error is >= 1e-2 (the first one included) are full
+            builds with FP32 small-bound quartets at the Schwarz cutoff loose_cutoff (1e-8:
+            ~1e-5 in F, 2.2x faster builds on (H2O)64 / cc-pVTZ);
+          * then delta builds (G += G(P - P_prev), FP32 allowed, mixed_delta_* cutoffs)
+            until the DIIS error of that Fock matrix is below switch_error (1e-4);
+          * then a full FP64 build at the engine's cutoffs on the current density, which
+            removes every loose, FP32 and delta error, and FP64 delta builds (delta_*
+            cutoffs) to convergence.  With final_full (default False) a converged run
+            makes one more full FP64 build and reports its energy."""
+        if incremental is None:
+            incremental = precision == "mixed"
+        if precision == "mixed":
+            if "loose_scale" not in kw:
+                s = self.engine.cutoffs[0]
+                kw["loose_scale"] = max(1.0, self.loose_cutoff / s) if s > 0 else 1.0
+            kw.setdefault("switch_error", self.switch_error)
+            kw.setdefault("final_full", self.final_full)
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         P0 = self.guess(guess, guess_model)

@@ -88,7 +88,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
             level_shift=0.3, level_shift_iters=3, damp=0.5, damp_iters=3,
             mixed_until=None, incremental=False, rebuild_every=8, incremental_from=1e-2,
             loose_scale=1.0, loose_until=None, rebuild_after_loose=True, diis_reset_on_switch=False,
-            log=None):
+            switch_error=None, final_full=True, keep_fock=False, log=None):
     """Run RHF to convergence.
 
     jk(P, mode) returns (J, K); mode is "full" or "delta" (a build from a density
@@ -105,6 +105,14 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
       usual cutoffs; otherwise delta builds go on from the loose G and the FP64 build at
       the precision switch is the first one without the loose error.
     diis_reset_on_switch: drop the DIIS history at the precision switch.
+    switch_error: if given, the precision switch comes when the DIIS error of the
+      low-precision Fock matrix falls below it (instead of |dE| < mixed_until): the
+      low-precision phase then converges the density as far as its own Fock matrix
+      allows, and the FP64 phase starts close to convergence.
+    keep_fock: keep each iteration's density and Fock matrix (host copies) in the history.
+    final_full: once converged on delta builds, make one more full FP64 build and accept
+      only its energy (False: the FP64 delta builds after the switch are trusted, so their
+      cutoffs must keep the energy within the convergence tolerance).
     incremental: update the 2e Fock part with builds on P - P_prev; after rebuild_every
       builds since the last full one, at the precision switch and once converged a full
       build is done, and convergence is only accepted on a full FP64 build.  While the
@@ -210,19 +218,22 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         E = E_new
         history.append(dict(iteration=it, energy=E, dE=dE, diis_error=err_max, mode=mode,
                             build_seconds=tb, seconds=time.perf_counter() - ti))
+        if keep_fock:
+            history[-1].update(P=P.detach().cpu().numpy().copy(), F=F.detach().cpu().numpy().copy())
         if log:
             log(f"iter {it:3d} E={E:.12f} dE={dE:+.3e} err={err_max:.3e} {mode} "
                 f"build {tb:.3f}s iteration {history[-1]['seconds']:.3f}s")
         done = abs(dE) < convergence and err_max < diis_tol
         if low:
-            if abs(dE) < mixed_until or done:
+            switch = (err_max < switch_error) if switch_error is not None else abs(dE) < mixed_until
+            if switch or done:
                 low = False
                 force_full = True
                 if diis_reset_on_switch:
                     diis_F.clear()
                     diis_E.clear()
             done = False
-        elif done and not last_full_fp64:
+        elif done and not last_full_fp64 and final_full:
             # win back the incremental error: a full FP64 build on the current density
             J, K = build(P, "full")
             G = J - 0.5 * K
