@@ -128,6 +128,20 @@ int tgpu_make_r_JK(tgpu_ctx *ctx, const double *P, double *J, double *K, int fla
  * wavefront-form workspace of f-shell bases), except the hipGraph of TGPU_JK_GRAPH. */
 int tgpu_make_r_JK_device(tgpu_ctx *ctx, const double *P, double *J, double *K, int flags);
 
+/* Allocation-free variant of tgpu_make_r_JK_device.  No counterpart in Tonto: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same results and flags (TGPU_JK_MIXED,
+ * TGPU_J_ONLY, TGPU_K_ONLY; not TGPU_JK_GRAPH); every device buffer the build writes
+ * besides J and K (the X/Y accumulators, shell-pair density maxima, counters and the
+ * f-shell workspace) is taken from one library-owned array of 64 MiB per GPU, which each
+ * GPU holds until the process ends.  Limit: 2 n_bf^2 doubles plus the shell-pair maxima
+ * and one workspace slot must fit 64 MiB (about n_bf <= 2000); larger systems get
+ * TGPU_E_UNSUPPORTED.  tgpu_schwarz must have run (TGPU_E_STATE otherwise) and the
+ * context's device must be current.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (two streams, two threads) race.  Asynchronous on the context's
+ * stream; tgpu_get_stats reports the build as usual. */
+int tgpu_make_r_JK_device_noalloc(tgpu_ctx *ctx, const double *P, double *J, double *K, int flags);
+
 /* UHF J and K from the alpha and beta densities Pa, Pb, host arrays of n_bf*n_bf
  * doubles (MOLECULE.FOCK:make_u_JK_engine, molecule.fock.foo:2558-2712): Ja, Jb, Ka, Kb
  * after the same post-processing as the RHF build (2688-2706).  As in Tonto, screening

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -273,6 +274,19 @@ struct DBuf {
     T *p = nullptr;
     size_t n = 0;
 };
+
+// ----------------------------------------------------------------------------------
+// Scratch of the allocation-free build (tgpu_make_r_JK_device_noalloc): one array per GPU,
+// held from the library's load on that GPU to the end of the process, shared by every
+// context on the GPU.  In host code `scratch` names a host-side stand-in, not device
+// memory: kernels, fills and copies get the device address that hipGetSymbolAddress
+// returns, looked up once per device when the first context is created there.
+// ----------------------------------------------------------------------------------
+#define TGPU_NOALLOC_SCRATCH_BYTES (64ull << 20)
+static __device__ __align__(256) unsigned char scratch[TGPU_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];  // device address of `scratch` on each device
 
 static const bool kHeavy[NQCLS] = {
 #define TGPU_HEAVY_FLAG(LP, LQ, LR, LS, H) H,
@@ -536,6 +550,24 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
     if (hipGetDevice(&c->device) != hipSuccess) {
         c->err = "no HIP device";
         return TGPU_E_HIP;
+    }
+    if (c->device < 0 || c->device >= kMaxDevices) {
+        c->err = "device ordinal " + std::to_string(c->device) + " beyond this build's " +
+                 std::to_string(kMaxDevices);
+        return TGPU_E_UNSUPPORTED;
+    }
+    {
+        // the allocation-free build's scratch address on this device (the current one)
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[c->device]) {
+            void *ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                c->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+                return TGPU_E_HIP;
+            }
+            g_scratch_ptr[c->device] = ptr;
+        }
     }
     c->n_mol = n_mol;
     if (const char *kb = std::getenv("TGPU_KROW_BUDGET")) c->krow_budget = std::atoi(kb);
@@ -1010,6 +1042,76 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
     HIPCHK(c, hipEventRecord(c->ev_out, c->gstream));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     c->timed = false;
+    return TGPU_OK;
+}
+
+__global__ void copy_u64_kernel(unsigned long long *__restrict__ dst, const unsigned long long *__restrict__ src,
+                                int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// The build of tgpu_make_r_JK_device with every buffer it writes besides J and K carved
+// out of this device's `scratch` array (256-byte aligned pieces): X, Y, P_max, the
+// per-molecule maxima, the class counters, and the rest as the wavefront-form workspace.
+// Nothing is allocated: no hipMalloc/hipFree, no library call that allocates.  Calls that
+// overlap on one device share the array and race.
+extern "C" int tgpu_make_r_JK_device_noalloc(tgpu_ctx *c, const double *P, double *J, double *K, int flags)
+{
+    if (!c || !P || !J || !K) return TGPU_E_ARG;
+    if (flags & ~(TGPU_JK_MIXED | TGPU_J_ONLY | TGPU_K_ONLY)) {
+        c->err = "tgpu_make_r_JK_device_noalloc: flags other than TGPU_JK_MIXED, TGPU_J_ONLY, TGPU_K_ONLY";
+        return TGPU_E_ARG;
+    }
+    if ((flags & TGPU_J_ONLY) && (flags & TGPU_K_ONLY)) {
+        c->err = "TGPU_J_ONLY and TGPU_K_ONLY are exclusive";
+        return TGPU_E_ARG;
+    }
+    auto up = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    const size_t b_mat = up((size_t)c->n_elem * sizeof(double));
+    const size_t b_pmax = up((size_t)c->n_pairs * sizeof(double));
+    const size_t b_scal = up(2 * (size_t)c->n_mol * sizeof(double));
+    const size_t b_cnt = up(3 * (size_t)NQCLS * sizeof(unsigned long long));
+    const size_t fixed = 2 * b_mat + b_pmax + b_scal + b_cnt;
+    const size_t ws_slot = workspace_need(c) * sizeof(double);  // one slot of the largest class
+    if (fixed + ws_slot > TGPU_NOALLOC_SCRATCH_BYTES) {
+        c->err = "tgpu_make_r_JK_device_noalloc: this system needs " + std::to_string(fixed + ws_slot) +
+                 " bytes of scratch (2 n_bf^2 doubles for " + std::to_string(c->n_elem) +
+                 " matrix elements, the shell-pair maxima and a workspace slot); the maximum is " +
+                 std::to_string(TGPU_NOALLOC_SCRATCH_BYTES) + " bytes";
+        return TGPU_E_UNSUPPORTED;
+    }
+    if (!c->have_schwarz) {
+        c->err = "tgpu_make_r_JK_device_noalloc: call tgpu_schwarz first (it allocates the Schwarz table)";
+        return TGPU_E_STATE;
+    }
+    int dev = -1;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev != c->device) {
+        c->err = "tgpu_make_r_JK_device_noalloc: the context's device is not the current one";
+        return TGPU_E_STATE;
+    }
+    unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[c->device]);
+    if (!base) {
+        c->err = "tgpu_make_r_JK_device_noalloc: no scratch address for this device";
+        return TGPU_E_STATE;
+    }
+    BuildBufs B;
+    size_t off = 0;
+    B.X = reinterpret_cast<double *>(base + off); off += b_mat;
+    B.Y = reinterpret_cast<double *>(base + off); off += b_mat;
+    B.Pmax = reinterpret_cast<double *>(base + off); off += b_pmax;
+    B.scal = reinterpret_cast<double *>(base + off); off += b_scal;
+    B.counters = reinterpret_cast<unsigned long long *>(base + off); off += b_cnt;
+    B.ws = ws_slot ? reinterpret_cast<double *>(base + off) : nullptr;
+    B.ws_doubles = ws_slot ? (long long)((TGPU_NOALLOC_SCRATCH_BYTES - off) / sizeof(double)) : 0;
+    const int kflags = flags & (TGPU_JK_MIXED | TGPU_J_ONLY | TGPU_K_ONLY);
+    if (int rc = enqueue_build(c, B, P, J, K, kflags, c->stream, true, true)) return rc;
+    // the context's own counters take this build's, so tgpu_get_stats reports it
+    copy_u64_kernel<<<1, 256, 0, c->stream>>>(c->d_counters.p, B.counters, 3 * NQCLS);
+    c->timed = true;
+    HIPCHK(c, hipGetLastError());
     return TGPU_OK;
 }
 

@@ -20,7 +20,7 @@ TGPU_K_ONLY = 8
 # exported symbols, in the order of include/tonto_gpu.h (plus the two extensions)
 EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_set_stream",
            "tgpu_set_cutoffs", "tgpu_set_fp32_threshold", "tgpu_schwarz", "tgpu_make_r_JK",
-           "tgpu_make_r_JK_device", "tgpu_make_u_JK", "tgpu_make_u_JK_device", "tgpu_get_stats",
+           "tgpu_make_r_JK_device", "tgpu_make_r_JK_device_noalloc", "tgpu_make_u_JK", "tgpu_make_u_JK_device", "tgpu_get_stats",
            "tgpu_one_electron",
            "tgpu_get_class_counters", "tgpu_n_class_pairs", "tgpu_set_profiling",
            "tgpu_get_class_times", "tgpu_create_batch", "tgpu_batch_layout", "tgpu_set_variant",
@@ -68,6 +68,7 @@ def lib():
         L.tgpu_schwarz.argtypes = [vp, vp]
         L.tgpu_make_r_JK.argtypes = [vp, dp, dp, dp, ctypes.c_int]
         L.tgpu_make_r_JK_device.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+        L.tgpu_make_r_JK_device_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_int]
         L.tgpu_make_u_JK.argtypes = [vp, dp, dp, dp, dp, dp, dp, ctypes.c_int]
         L.tgpu_make_u_JK_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         L.tgpu_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -199,6 +200,14 @@ class JKEngine:
         self._check(lib().tgpu_make_r_JK_device(self._ctx, ctypes.c_void_p(P_ptr),
                                                 ctypes.c_void_p(J_ptr), ctypes.c_void_p(K_ptr),
                                                 flags))
+
+    def make_r_JK_device_noalloc(self, P_ptr, J_ptr, K_ptr, mixed=False, only=None):
+        """The allocation-free build: device pointers, asynchronous on the stream, its
+        scratch buffers in the library's per-GPU array (TGPUError if they do not fit)."""
+        flags = (TGPU_JK_MIXED if mixed else 0) | {None: 0, "J": TGPU_J_ONLY, "K": TGPU_K_ONLY}[only]
+        self._check(lib().tgpu_make_r_JK_device_noalloc(self._ctx, ctypes.c_void_p(P_ptr),
+                                                        ctypes.c_void_p(J_ptr), ctypes.c_void_p(K_ptr),
+                                                        flags))
 
     def autotune(self, P_ptr, mixed=False):
         n = lib().tgpu_n_class_pairs()
