@@ -649,18 +649,19 @@ static void make_r_JK_impl(const osys_t *s, const ocut_t *cut, const double *P, 
 /* post-processing (molecule.fock.foo:1485-1500, 2688-2706 for UHF):
  * weight_diagonal_blocks(J,2); blockwise_symmetric_fold(K); J = 2J; symmetric_reflect
  * (lower -> upper) */
-static void post_jk(const osys_t *s, double *J, double *K)
+/* post-processing over shell blocks given by first function and size (Cartesian for the
+ * engine, spherical for the direct build) */
+static void post_jk_blocks(int ns, const int *first, const int *cnt, int n, double *J, double *K)
 {
-    int ns = s->n_shell, n = s->n_bf;
     for (int a = 0; a < ns; a++) {
-        int fa = s->first_bf[a], na = ncomp(s->l[a]);
+        int fa = first[a], na = cnt[a];
         for (int i = 0; i < na; i++)
             for (int j = 0; j < na; j++) J[(size_t)(fa + i) * n + fa + j] *= 2.0;
     }
     double *tmp = malloc(sizeof(double) * 100);
     for (int a = 0; a < ns; a++)
         for (int b = 0; b <= a; b++) {
-            int fa = s->first_bf[a], na = ncomp(s->l[a]), fb = s->first_bf[b], nb = ncomp(s->l[b]);
+            int fa = first[a], na = cnt[a], fb = first[b], nb = cnt[b];
             for (int i = 0; i < na; i++)
                 for (int j = 0; j < nb; j++) tmp[i * nb + j] = K[(size_t)(fb + j) * n + fa + i];
             for (int i = 0; i < na; i++)
@@ -673,6 +674,14 @@ static void post_jk(const osys_t *s, double *J, double *K)
             J[(size_t)j * n + i] = J[(size_t)i * n + j];
             K[(size_t)j * n + i] = K[(size_t)i * n + j];
         }
+}
+
+static void post_jk(const osys_t *s, double *J, double *K)
+{
+    int *cnt = malloc(sizeof(int) * s->n_shell);
+    for (int a = 0; a < s->n_shell; a++) cnt[a] = ncomp(s->l[a]);
+    post_jk_blocks(s->n_shell, s->first_bf, cnt, s->n_bf, J, K);
+    free(cnt);
 }
 
 /* ---------------- one-electron integrals (molecule.ints.foo:34-415) ------------ */
@@ -893,4 +902,251 @@ int orc_pair_prims(const osys_t *s, int a, int b, int skip, double pair_cut)
     pairdat_t pd;
     make_pair(s, a, b, skip, pair_cut, &pd);
     return pd.n;
+}
+
+/* ---------------- spherical basis: MOLECULE.FOCK:make_r_JK_direct ---------------------
+ * (molecule.fock.foo:1235-1363) with Tonto's spherical integrals: the Cartesian block of
+ * each quartet turned into real solid harmonics index by index for shells with l >= 2
+ * (SHELL1QUARTET:make_ERI / change_to_spherical, shell1quartet.foo:433-458, 505-563),
+ * screening by Schwarz_inequality_test (molecule.fock.foo:3454-3495: the largest of the
+ * six density blocks ab, cd, ac, ad, bc, bd; skip below TOL(20) or when
+ * Pmax*I_ab*I_cd > cutoff fails), one decision for J and K together, max_I and P_max over
+ * spherical blocks (3601-3633, 3705-3752).  The solid harmonics are written out in closed
+ * form (real solid harmonics, Molden order M = 0, +1, -1, +2, -2, ..., normalised like
+ * x^l), independently of the product's recurrence. */
+#define SQ3 1.7320508075688772935
+static double g_T2[6][5], g_T3[10][7];
+static int g_sph_ready = 0;
+
+static void sph_tables(void)
+{
+    if (g_sph_ready) return;
+    const double a = sqrt(3.0 / 8.0), b = sqrt(5.0 / 8.0), c = sqrt(15.0);
+    memset(g_T2, 0, sizeof g_T2);
+    memset(g_T3, 0, sizeof g_T3);
+    /* d: xx yy zz xy xz yz */
+    g_T2[0][0] = -0.5; g_T2[1][0] = -0.5; g_T2[2][0] = 1.0;  /* z^2 - (x^2+y^2)/2 */
+    g_T2[4][1] = SQ3;                                        /* sqrt3 xz */
+    g_T2[5][2] = SQ3;                                        /* sqrt3 yz */
+    g_T2[0][3] = SQ3 / 2; g_T2[1][3] = -SQ3 / 2;             /* sqrt3/2 (x^2 - y^2) */
+    g_T2[3][4] = SQ3;                                        /* sqrt3 xy */
+    /* f: xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz */
+    g_T3[2][0] = 1.0; g_T3[4][0] = -1.5; g_T3[6][0] = -1.5;  /* z (2z^2 - 3x^2 - 3y^2)/2 */
+    g_T3[0][1] = -a; g_T3[5][1] = -a; g_T3[7][1] = 4 * a;    /* sqrt(3/8) x (4z^2 - x^2 - y^2) */
+    g_T3[1][2] = -a; g_T3[3][2] = -a; g_T3[8][2] = 4 * a;    /* sqrt(3/8) y (4z^2 - x^2 - y^2) */
+    g_T3[4][3] = c / 2; g_T3[6][3] = -c / 2;                 /* sqrt15/2 z (x^2 - y^2) */
+    g_T3[9][4] = c;                                          /* sqrt15 xyz */
+    g_T3[0][5] = b; g_T3[5][5] = -3 * b;                     /* sqrt(5/8) x (x^2 - 3y^2) */
+    g_T3[3][6] = 3 * b; g_T3[1][6] = -b;                     /* sqrt(5/8) y (3x^2 - y^2) */
+    g_sph_ready = 1;
+}
+
+static int nsph(int l) { return l >= 2 ? 2 * l + 1 : ncomp(l); }
+
+void orc_solid_harmonics(int l, double *out)  /* ncomp(l) x (2l+1), row-major */
+{
+    sph_tables();
+    for (int i = 0; i < ncomp(l); i++)
+        for (int m = 0; m < 2 * l + 1; m++)
+            out[i * (2 * l + 1) + m] = l == 2 ? g_T2[i][m] : l == 3 ? g_T3[i][m] : (l < 2 && i == m ? 1.0 : 0.0);
+}
+
+static double tval(int l, int i, int m) { return l == 2 ? g_T2[i][m] : l == 3 ? g_T3[i][m] : (i == m); }
+
+/* in: [na][nb][nc][nd] Cartesian; out: the same with every l >= 2 index spherical */
+static void cart_to_sph4(const int *l4, const double *in, double *out, double *t0, double *t1)
+{
+    int dims[4], dimo[4];
+    for (int k = 0; k < 4; k++) { dims[k] = ncomp(l4[k]); dimo[k] = nsph(l4[k]); }
+    const double *cur = in;
+    int cd[4] = {dims[0], dims[1], dims[2], dims[3]};
+    for (int k = 0; k < 4; k++) {
+        double *nxt = k == 3 ? out : (k % 2 ? t1 : t0);
+        if (l4[k] < 2) {
+            long n = (long)cd[0] * cd[1] * cd[2] * cd[3];
+            memcpy(nxt, cur, sizeof(double) * n);
+            cur = nxt;
+            continue;
+        }
+        int pre = 1, post = 1;
+        for (int q = 0; q < k; q++) pre *= cd[q];
+        for (int q = k + 1; q < 4; q++) post *= cd[q];
+        const int ni = cd[k], no = dimo[k];
+        for (int p = 0; p < pre; p++)
+            for (int m = 0; m < no; m++)
+                for (int r = 0; r < post; r++) {
+                    double acc = 0.0;
+                    for (int i = 0; i < ni; i++) acc += tval(l4[k], i, m) * cur[((long)p * ni + i) * post + r];
+                    nxt[((long)p * no + m) * post + r] = acc;
+                }
+        cd[k] = no;
+        cur = nxt;
+    }
+}
+
+static long sph_quartet(const osys_t *s, int a, int b, int c, int d, int skip, const ocut_t *cut, double *out,
+                        qws_t *ws, double *cbuf)
+{
+    long np = quartet(s, a, b, c, d, skip, cut, cbuf, ws);
+    const int l4[4] = {s->l[a], s->l[b], s->l[c], s->l[d]};
+    cart_to_sph4(l4, cbuf, out, ws->t0, ws->t1);
+    return np;
+}
+
+typedef struct {
+    const osys_t *s;
+    const ocut_t *cut;
+    const int *sf;            /* first spherical function of each shell */
+    int nsf;                  /* spherical functions */
+    const double *P, *max_I, *Pmax;
+    double P_max, IP_max;
+    double **Jt, **Kt, **Ibuf, **Cbuf;
+    qws_t **ws;
+    long (*st)[5];
+} direct_job;
+
+static void schwarz_sph_item(void *arg, long item, int worker)
+{
+    (void)worker;
+    void **A = (void **)arg;
+    const osys_t *s = (const osys_t *)A[0];
+    const ocut_t *cut = (const ocut_t *)A[1];
+    double *max_I = (double *)A[2];
+    int a = (int)item;
+    double *buf = malloc(sizeof(double) * 10000), *cb = malloc(sizeof(double) * 10000);
+    qws_t *ws = malloc(sizeof(qws_t));
+    for (int b = 0; b <= a; b++) {
+        sph_quartet(s, a, b, a, b, 0, cut, buf, ws, cb);
+        const int na = nsph(s->l[a]), nb = nsph(s->l[b]);
+        double m = 0.0;
+        for (int ia = 0; ia < na; ia++)
+            for (int ib = 0; ib < nb; ib++) {
+                double v = fabs(buf[((ia * nb + ib) * na + ia) * nb + ib]);
+                if (v > m) m = v;
+            }
+        max_I[(long)a * (a + 1) / 2 + b] = sqrt(m);
+    }
+    free(ws); free(buf); free(cb);
+}
+
+/* max_I over spherical (ab|ab) blocks (make_max_abab_integrals with a spherical basis) */
+void orc_schwarz_sph(const osys_t *s, double *max_I)
+{
+    comp_tables();
+    sph_tables();
+    ocut_t cut = {0, 0, 0, 1e-6, 0};
+    void *A[3] = {(void *)s, &cut, max_I};
+    parallel_for(s->n_shell, orc_max_threads(), schwarz_sph_item, A);
+}
+
+static void direct_item(void *arg, long ab, int w)
+{
+    direct_job *jb = (direct_job *)arg;
+    const osys_t *s = jb->s;
+    const ocut_t *cut = jb->cut;
+    const double *max_I = jb->max_I, *Pmax = jb->Pmax, *P = jb->P;
+    const int n = jb->nsf;
+    double *Jl = jb->Jt[w], *Kl = jb->Kt[w], *I = jb->Ibuf[w];
+    long *st = jb->st[w];
+    if (max_I[ab] * jb->IP_max < cut->schwarz) return;
+    int a, b;
+    decode(ab, &a, &b);
+    for (long cd = 0; cd <= ab; cd++) {
+        if (max_I[ab] * max_I[cd] * jb->P_max < cut->schwarz) continue;
+        int c, d;
+        decode(cd, &c, &d);
+        /* Schwarz_inequality_test (molecule.fock.foo:3454-3495) */
+        double Pm = Pmax[ab];
+        const long blk[5] = {cd, pidx(a, c), pidx(a, d), pidx(b, c), pidx(b, d)};
+        for (int k = 0; k < 5; k++)
+            if (Pmax[blk[k]] > Pm) Pm = Pmax[blk[k]];
+        if (Pm < 1e-20) continue;
+        if (!(Pm * max_I[ab] * max_I[cd] > cut->schwarz)) continue;
+        double f = (a == b) ? 0.5 : 1.0;
+        if (c == d) f *= 0.5;
+        if (a == c && b == d) f *= 0.5;
+        const int any_same = !uses_skip_lists(s, a, b, c, d);
+        long npq = sph_quartet(s, a, b, c, d, !any_same, cut, I, jb->ws[w], jb->Cbuf[w]);
+        st[0]++; st[1] += npq; st[4] += !any_same;
+        const int na = nsph(s->l[a]), nb = nsph(s->l[b]), nc = nsph(s->l[c]), nd = nsph(s->l[d]);
+        const int fa = jb->sf[a], fb = jb->sf[b], fc = jb->sf[c], fd = jb->sf[d];
+        for (int ia = 0; ia < na; ia++)
+            for (int ib = 0; ib < nb; ib++)
+                for (int ic = 0; ic < nc; ic++)
+                    for (int id = 0; id < nd; id++) {
+                        const double v = f * I[((ia * nb + ib) * nc + ic) * nd + id];
+                        const int i = fa + ia, j = fb + ib, k = fc + ic, l = fd + id;
+                        Jl[(size_t)i * n + j] += v * P[(size_t)l * n + k];
+                        Jl[(size_t)k * n + l] += v * P[(size_t)j * n + i];
+                        Kl[(size_t)i * n + k] += v * P[(size_t)l * n + j];
+                        Kl[(size_t)i * n + l] += v * P[(size_t)k * n + j];
+                        Kl[(size_t)j * n + k] += v * P[(size_t)l * n + i];
+                        Kl[(size_t)j * n + l] += v * P[(size_t)k * n + i];
+                    }
+    }
+}
+
+/* J and K in the spherical basis (n_sph x n_sph, row-major) from the spherical density */
+void orc_make_r_JK_direct(const osys_t *s, const ocut_t *cut, const double *P, const double *max_I_in,
+                          double *J, double *K, long *stats)
+{
+    comp_tables();
+    sph_tables();
+    const int ns = s->n_shell;
+    int *sf = malloc(sizeof(int) * (ns + 1));
+    sf[0] = 0;
+    for (int a = 0; a < ns; a++) sf[a + 1] = sf[a] + nsph(s->l[a]);
+    const int n = sf[ns];
+    const long npair = (long)ns * (ns + 1) / 2;
+    double *max_I = malloc(sizeof(double) * npair);
+    if (max_I_in) memcpy(max_I, max_I_in, sizeof(double) * npair);
+    else orc_schwarz_sph(s, max_I);
+    double *Pmax = malloc(sizeof(double) * npair);
+    for (int a = 0; a < ns; a++)
+        for (int b = 0; b <= a; b++) {
+            double m = 0.0;
+            for (int i = 0; i < nsph(s->l[a]); i++)
+                for (int j = 0; j < nsph(s->l[b]); j++) {
+                    const double v = fabs(P[(size_t)(sf[a] + i) * n + sf[b] + j]);
+                    if (v > m) m = v;
+                }
+            Pmax[(long)a * (a + 1) / 2 + b] = m;
+        }
+    double P_max = 0.0, I_max = 0.0;
+    for (long i = 0; i < npair; i++) {
+        if (Pmax[i] > P_max) P_max = Pmax[i];
+        if (max_I[i] > I_max) I_max = max_I[i];
+    }
+    const int nthr = orc_max_threads();
+    direct_job jb;
+    jb.s = s; jb.cut = cut; jb.sf = sf; jb.nsf = n; jb.P = P; jb.max_I = max_I; jb.Pmax = Pmax;
+    jb.P_max = P_max; jb.IP_max = I_max * P_max;
+    jb.Jt = malloc(sizeof(double *) * nthr); jb.Kt = malloc(sizeof(double *) * nthr);
+    jb.Ibuf = malloc(sizeof(double *) * nthr); jb.Cbuf = malloc(sizeof(double *) * nthr);
+    jb.ws = malloc(sizeof(qws_t *) * nthr);
+    jb.st = calloc(nthr, sizeof(long[5]));
+    for (int t = 0; t < nthr; t++) {
+        jb.Jt[t] = calloc((size_t)n * n, sizeof(double));
+        jb.Kt[t] = calloc((size_t)n * n, sizeof(double));
+        jb.Ibuf[t] = malloc(sizeof(double) * 10000);
+        jb.Cbuf[t] = malloc(sizeof(double) * 10000);
+        jb.ws[t] = malloc(sizeof(qws_t));
+    }
+    parallel_for(npair, nthr, direct_item, &jb);
+    memset(J, 0, sizeof(double) * n * n);
+    memset(K, 0, sizeof(double) * n * n);
+    long st[5] = {0, 0, 0, 0, 0};
+    for (int t = 0; t < nthr; t++) {
+        for (long i = 0; i < (long)n * n; i++) { J[i] += jb.Jt[t][i]; K[i] += jb.Kt[t][i]; }
+        for (int k = 0; k < 5; k++) st[k] += jb.st[t][k];
+        free(jb.Jt[t]); free(jb.Kt[t]); free(jb.Ibuf[t]); free(jb.Cbuf[t]); free(jb.ws[t]);
+    }
+    free(jb.Jt); free(jb.Kt); free(jb.Ibuf); free(jb.Cbuf); free(jb.ws); free(jb.st);
+    /* post-processing over spherical shell blocks (molecule.fock.foo:1350-1361) */
+    int *cnt = malloc(sizeof(int) * ns);
+    for (int a = 0; a < ns; a++) cnt[a] = nsph(s->l[a]);
+    post_jk_blocks(ns, sf, cnt, n, J, K);
+    free(cnt);
+    if (stats) memcpy(stats, st, sizeof st);
+    free(max_I); free(Pmax); free(sf);
 }

@@ -48,6 +48,10 @@ def lib():
         _lib.orc_pair_prims.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_double]
         _lib.orc_pair_prims.restype = ctypes.c_int
         _lib.orc_set_plant.argtypes = [ctypes.c_int]
+        _lib.orc_make_r_JK_direct.argtypes = [ctypes.c_void_p, ctypes.c_void_p, d, ctypes.c_void_p, d, d,
+                                              ctypes.POINTER(ctypes.c_long)]
+        _lib.orc_schwarz_sph.argtypes = [ctypes.c_void_p, d]
+        _lib.orc_solid_harmonics.argtypes = [ctypes.c_int, d]
     return _lib
 
 
@@ -120,6 +124,31 @@ class OracleSystem:
         lib().orc_quartet_audit(ctypes.byref(self.s), ctypes.byref(cut), _dp(P), _dp(mi),
                                 code.ctypes.data, _dp(maxabs))
         return code, maxabs
+
+    @property
+    def n_sph(self):
+        return sum(2 * s.l + 1 if s.l >= 2 else s.n_comp for s in self.mol.shells)
+
+    def schwarz_sph(self):
+        """max_I from the spherical (ab|ab) blocks."""
+        out = np.zeros(self.n_pairs)
+        lib().orc_schwarz_sph(ctypes.byref(self.s), _dp(out))
+        return out
+
+    def make_r_JK_direct(self, P, schwarz=1e-9, pair=1e-6, max_I=None, return_stats=False):
+        """Spherical-basis J/K (make_r_JK_direct): P, J, K are n_sph x n_sph."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n = self.n_sph
+        assert P.shape == (n, n)
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        cut = _OCut(schwarz, 0.0, 0.0, pair, 0)
+        st = (ctypes.c_long * 5)()
+        mi = None if max_I is None else np.ascontiguousarray(max_I, dtype=np.float64)
+        lib().orc_make_r_JK_direct(ctypes.byref(self.s), ctypes.byref(cut), _dp(P),
+                                   None if mi is None else mi.ctypes.data, _dp(J), _dp(K), st)
+        if return_stats:
+            return J, K, dict(quartets=st[0], prim_quartets=st[1], skip_lists=st[4])
+        return J, K
 
     def pair_prims(self, a, b, skip, pair=1e-6):
         return lib().orc_pair_prims(ctypes.byref(self.s), a, b, int(skip), pair)
@@ -204,6 +233,13 @@ def boys_quad(m, X):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def solid_harmonics(l):
+    """The oracle's closed-form Cartesian -> real solid harmonic matrix (ncomp(l), 2l+1)."""
+    out = np.zeros(((l + 1) * (l + 2) // 2) * (2 * l + 1))
+    lib().orc_solid_harmonics(int(l), _dp(out))
+    return out.reshape((l + 1) * (l + 2) // 2, 2 * l + 1)
 
 
 def set_plant(k):

@@ -92,3 +92,28 @@ def test_alpha_beta_split_against_full_tensor(oracle_mod):
     K0b = np.einsum("prqs,rs->pq", eri, Pb)
     assert np.max(np.abs(Ka - K0a)) < 1e-12 * np.max(np.abs(K0a))
     assert np.max(np.abs(Kb - K0b)) < 1e-12 * np.max(np.abs(K0b))
+
+
+def test_uhf_scf_reaches_g09_cation(oracle_mod):
+    """H2O+ UHF/cc-pVDZ (6D), Tonto's water geometry: g09 -75.6327038760
+    (scripts/check_dft_reference.py:72; Tonto agreed to 2.0e-10 there).  The oracle's
+    UHF J/K (make_u_JK_engine's split included) in tonto_amd.scf.run_uhf."""
+    import torch
+    from tonto_amd.geometry import tonto_water
+    from tonto_amd.scf import run_uhf
+    m = tonto_water("cc-pVDZ", charge=1)
+    assert m.n_electrons == 9
+    o = oracle_mod.OracleSystem(m)
+    S, T, V = o.one_electron()
+    mi = o.schwarz()
+
+    def jk(Pa, Pb):
+        return tuple(torch.from_numpy(x) for x in o.make_u_JK(Pa.numpy(), Pb.numpy(), max_I=mi))
+    r = run_uhf(torch.from_numpy(T + V), torch.from_numpy(S), 5, 4, m.nuclear_repulsion(), jk,
+                convergence=1e-10, diis_tol=1e-7)
+    assert r.converged
+    assert abs(r.energy - (-75.6327038760)) < 1e-9
+    # <S^2> of a UHF doublet is slightly above 3/4 (spin contamination), and the alpha and
+    # beta densities hold 5 and 4 electrons
+    Sm = torch.from_numpy(S)
+    assert abs(float(torch.sum(r.Pa * Sm)) - 5) < 1e-10 and abs(float(torch.sum(r.Pb * Sm)) - 4) < 1e-10

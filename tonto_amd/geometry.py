@@ -76,12 +76,13 @@ def water_cluster(n, basis_name, seed=0, box=None):
     return Molecule.from_angstrom(symbols, xyz, basis_name)
 
 
-def tonto_water(basis_name):
-    """The single water of Tonto's `tests/short/h2o_rhf_*` jobs (Angstrom input)."""
+def tonto_water(basis_name, charge=0):
+    """The single water of Tonto's `tests/short/h2o_rhf_*` jobs (Angstrom input); the
+    cation (charge 1) of scripts/check_dft_reference.py uses the same geometry."""
     return Molecule.from_angstrom(
         ["O", "H", "H"],
         [[-0.028260, 0.000029, 0.0], [0.599917, 0.767685, 0.0], [0.599941, -0.767609, 0.0]],
-        basis_name)
+        basis_name, charge=charge)
 
 
 def cn_hcn_dimer(basis_name="cc-pVDZ"):

@@ -250,3 +250,101 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     wall = time.perf_counter() - t0
     return SCFResult(energy=E, converged=converged, iterations=it, P=P, F=F, C=C, eps=eps,
                      history=history, wall=wall, n_builds=n_builds, build_seconds=build_s)
+
+
+def run_uhf(h, S, n_alpha, n_beta, e_nuc, jk, Pa0=None, Pb0=None, convergence=1e-8, diis_tol=1e-6,
+            max_iterations=100, diis_keep=8, diis_start=3, level_shift=0.3, level_shift_iters=3,
+            damp=0.5, damp_iters=3, log=None):
+    """Unrestricted Hartree-Fock to convergence, the spin-resolved form of run_rhf.
+
+    jk(Pa, Pb) returns (Ja, Jb, Ka, Kb) as Tonto's make_u_JK_engine leaves them
+    (molecule.fock.foo:2558-2712); the Fock matrices are Tonto's
+    F.a = h + J.a + J.b - K.a, F.b = h + J.a + J.b - K.b (molecule.fock.foo:247-300),
+    the energy E = 1/2 tr[(Pa + Pb) h + Pa F.a + Pb F.b] + V_NN.  One DIIS for both spins
+    (the error matrices of both in one inner product, the same coefficients for F.a and
+    F.b), and damping and level shift of the first iterations as in run_rhf.  Converged
+    when |dE| and the larger DIIS error are below their tolerances."""
+    dt = h.dtype
+    X = _sym_orth(S)
+    n = h.shape[0]
+    eye = torch.eye(n, dtype=dt, device=h.device)
+
+    def occupy(F, n_occ):
+        e, Cp = torch.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        return C[:, :n_occ] @ C[:, :n_occ].T, e, C
+
+    if Pa0 is None or Pb0 is None:
+        Pa, _, _ = occupy(h, n_alpha)
+        Pb, _, _ = occupy(h, n_beta)
+    else:
+        Pa, Pb = Pa0.clone(), Pb0.clone()
+
+    def fock(Pa, Pb):
+        Ja, Jb, Ka, Kb = jk(Pa, Pb)
+        J = Ja + Jb
+        Fa, Fb = h + J - Ka, h + J - Kb
+        E = 0.5 * torch.sum((Pa + Pb) * h + Pa * Fa + Pb * Fb).item() + e_nuc
+        return Fa, Fb, E
+
+    t0 = time.perf_counter()
+    Fa, Fb, E = fock(Pa, Pb)
+    hist_F, hist_E = [], []
+    history = []
+    converged = False
+    it = 0
+    while it < max_iterations:
+        it += 1
+        ea = X.T @ (Fa @ Pa @ S - S @ Pa @ Fa) @ X
+        eb = X.T @ (Fb @ Pb @ S - S @ Pb @ Fb) @ X
+        err = max(ea.abs().max().item(), eb.abs().max().item())
+        hist_F.append((Fa, Fb))
+        hist_E.append((ea, eb))
+        if len(hist_F) > diis_keep:
+            hist_F.pop(0)
+            hist_E.pop(0)
+        Fa_u, Fb_u = Fa, Fb
+        if it >= diis_start and len(hist_F) >= 2:
+            m = len(hist_F)
+            B = torch.empty((m + 1, m + 1), dtype=dt, device=h.device)
+            for i in range(m):
+                for j in range(i + 1):
+                    B[i, j] = B[j, i] = (torch.sum(hist_E[i][0] * hist_E[j][0]) +
+                                         torch.sum(hist_E[i][1] * hist_E[j][1]))
+            B[m, :] = -1.0
+            B[:, m] = -1.0
+            B[m, m] = 0.0
+            rhs = torch.zeros(m + 1, dtype=dt, device=h.device)
+            rhs[m] = -1.0
+            try:
+                c = torch.linalg.solve(B, rhs)[:m]
+                Fa_u = sum(ci * F[0] for ci, F in zip(c, hist_F))
+                Fb_u = sum(ci * F[1] for ci, F in zip(c, hist_F))
+            except RuntimeError:
+                pass
+        Pn = []
+        for F_u, P, n_occ in ((Fa_u, Pa, n_alpha), (Fb_u, Pb, n_beta)):
+            if level_shift and it < level_shift_iters:
+                # virtual space raised by the shift: F + shift (S - S P S) in the S^-1/2 basis
+                Xi = S @ X
+                Fp = X.T @ F_u @ X + level_shift * (eye - Xi @ P @ Xi.T)
+                e, Cp = torch.linalg.eigh(Fp)
+                C = X @ Cp
+                Pnew = C[:, :n_occ] @ C[:, :n_occ].T
+            else:
+                Pnew, _, _ = occupy(F_u, n_occ)
+            if damp and it < damp_iters:
+                Pnew = damp * P + (1.0 - damp) * Pnew
+            Pn.append(Pnew)
+        Pa, Pb = Pn
+        Fa, Fb, E_new = fock(Pa, Pb)
+        dE = E_new - E
+        E = E_new
+        history.append(dict(iteration=it, energy=E, dE=dE, diis_error=err))
+        if log:
+            log(f"uhf iter {it:3d} E={E:.12f} dE={dE:+.3e} err={err:.3e}")
+        if abs(dE) < convergence and err < diis_tol:
+            converged = True
+            break
+    return SCFResult(energy=E, converged=converged, iterations=it, Pa=Pa, Pb=Pb, Fa=Fa, Fb=Fb,
+                     history=history, wall=time.perf_counter() - t0)
