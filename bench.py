@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the direct-SCF J/K hot path on one MI355X.
 
-Workload: (H2O)32 RHF / 6-311G(d,p), Cartesian d (992 basis functions), water clusters
-generated by tonto_amd.geometry (seed 0), Schwarz cutoff 1e-9, J/K density cutoffs 1e-9,
-primitive-pair cutoff 1e-6; quartets with max_I(ab) max_I(cd) < 1e-6 run in FP32.
+Workload (the headline): (H2O)64 RHF / cc-pVTZ with f on O, Cartesian (4160 basis
+functions), water cluster from tonto_amd.geometry (seed 0), Schwarz cutoff 1e-11, J/K
+density cutoffs 1e-9, primitive-pair cutoff 1e-6; quartets with max_I(ab) max_I(cd) < 1e-6
+run in FP32 in the mixed builds.
 
-1. FP64 SCF and mixed-precision SCF (FP32 small-bound quartets + incremental builds, FP64
-   builds once |dE| < 1e-5 and for the final iterations): wall time, iterations, energy
-   difference.
-2. Timed steps: one step = one J/K build (mixed precision) at the converged density,
-   P, J and K resident in HBM; W warm-up steps, then K timed steps between device
+1. Warm-up (not timed): promolecule guess (atomic SCFs) and one loose build of each
+   precision, so both SCF runs start from loaded kernels and cached atomic densities.
+2. FP64 SCF (full FP64 build every iteration) and mixed SCF (loose first builds, FP32
+   small-bound quartets, delta builds, FP64 from DIIS error 1e-4 on): wall time,
+   iterations, energy difference.  The FP64 build time comes from the FP64 SCF.
+3. Timed steps: one step = one mixed-precision J/K build at the converged density with P,
+   J and K resident in HBM; W warm-up steps, then K timed steps between device
    synchronisations.  value = contracted shell quartets computed per second.
-3. Roofline of the dominant class kernel from per-class HIP events recorded on the
-   library's stream during the timed steps and the algorithmic FLOPs of tonto_amd.flops.
-4. CPU baseline: the oracle's J/K build (oracle/oracle.c) on a bounded sample of the same
-   build (every k-th bra shell pair), on all host cores.
+4. Roofline of the dominant class kernel (FP64 vector ALU bound) from per-class HIP
+   events recorded on the library's stream during the timed steps and the algorithmic
+   FLOPs of tonto_amd.flops.
+5. CPU baseline: the oracle's FP64 J/K build (oracle/oracle.c) on a bounded sample of the
+   same build (every k-th bra shell pair) at the converged density, on all host cores and
+   on one.
+6. Side legs (configs of the other workloads, small): (H2O)32 / 6-311G(d,p) FP64 vs mixed
+   SCF, 256 batched clusters, the learned guess on held-out (H2O)20 clusters.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 import argparse
 import json
@@ -31,41 +38,54 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SCF wall-time & ERI quartets/s, mixed vs FP64 GPU at |ΔE|≤1e-8 Eh"
-FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published
-N_WATER, BASIS, SEED = 32, "6-311G(d,p)", 0
-SCHWARZ, FP32_THR = 1e-9, 1e-6
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and FP64 matrix), AMD published
+N_WATER, BASIS, SEED = 64, "cc-pVTZ", 0
+SCHWARZ, FP32_THR = 1e-11, 1e-6
 # HBM bytes per build of the dominant kernel on this workload, from rocprofv3 FETCH_SIZE
-# and WRITE_SIZE passes scaled by the same counters on a copy of known size
-# (profiles/jk_w32_hbm_traffic.md; sum over the class pair's 16 launches of one build)
-TRAFFIC = {"jk class (10|00) mixed": {"bytes": 8.869e9, "read": 4.331e9, "written": 4.538e9}}
+# and WRITE_SIZE passes scaled by the same counters on a copy of known size (the
+# profiles/ traffic summary of this workload; sum over the class pair's launches of one
+# build)
+TRAFFIC = {}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mol, P, target_s):
-    """Oracle J/K build on a bounded sample of the same build, all host cores."""
+def cpu_baseline(mol, P, max_I, target_s):
+    """The oracle's FP64 J/K build on a bounded sample of the same build (every k-th bra
+    shell pair, k chosen for about target_s seconds on all host cores), at the same
+    cutoffs and density; timed on all cores and on one (the same sample)."""
     import oracle.oracle as O
     o = O.OracleSystem(mol)
     cores = O.max_threads()
-    max_I = o.schwarz()
     npair = mol.n_shell * (mol.n_shell + 1) // 2
-    stride = max(1, npair // 128)
+    stride = max(1, npair // 64)
     t = time.perf_counter()
-    st = o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
-    dt = time.perf_counter() - t
-    full_est = dt * stride
+    o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
+    full_est = (time.perf_counter() - t) * stride
     stride = max(1, int(math.ceil(full_est / target_s)))
     t = time.perf_counter()
     st = o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
     dt = time.perf_counter() - t
+    # one thread on a sample a tenth as large (every 10k-th pair of the same list)
+    O.set_threads(1)
+    try:
+        t = time.perf_counter()
+        st1 = o.make_r_JK_sample(P, 10 * stride, schwarz=SCHWARZ, max_I=max_I)
+        dt1 = time.perf_counter() - t
+    finally:
+        O.set_threads(cores)
     return {"value": st["quartets"] / dt, "unit": "quartets/s", "cores": cores, "kind": "port",
             "sample": f"FP64 J/K build of the same workload at the converged density restricted to "
                       f"every {stride}-th bra shell pair ({st['quartets']} quartets, "
                       f"{st['prim_quartets']} primitive quartets, {dt:.1f} s); oracle/oracle.c, "
                       f"{cores} pthreads",
-            "seconds": dt}
+            "one_thread": {"value": st1["quartets"] / dt1, "unit": "quartets/s", "cores": 1,
+                           "sample": f"every {10 * stride}-th bra shell pair ({st1['quartets']} "
+                                     f"quartets, {dt1:.1f} s)"},
+            "full_build_estimate_s": dt * stride,
+            "seconds": dt + dt1}
 
 
 def batch_leg(steps, log_fn):
@@ -158,56 +178,48 @@ def guess_leg(n_clusters, log_fn):
             "max_abs_dE": float(np.max(np.abs(r["dE"]))), "dE_within_1e-8": bool(np.max(np.abs(r["dE"])) <= 1e-8)}
 
 
-def scale_leg(log_fn, delta_scale=None):
-    """(H2O)64 / cc-pVTZ (f on O), Schwarz cutoff 1e-11: FP64 SCF and mixed-precision SCF
-    (incremental builds) wall times and the energy difference."""
+def config2_leg(log_fn):
+    """(H2O)32 / 6-311G(d,p) (992 Cartesian functions), Schwarz 1e-9: FP64 SCF and mixed SCF
+    wall times and energy difference, and the mixed build's max |dF| at the converged
+    density against the FP64 build."""
     import torch
     from tonto_amd.driver import GPUSCF
     from tonto_amd.geometry import water_cluster
-    mol = water_cluster(64, "cc-pVTZ", seed=SEED)
-    log_fn(f"scale leg: (H2O)64 cc-pVTZ n_bf={mol.n_bf}")
-    scf = GPUSCF(mol, schwarz=1e-11, fp32_threshold=FP32_THR)
-    if delta_scale is not None:
-        scf.delta_cutoff = delta_scale * scf.engine.cutoffs[0]
-    r64 = scf.run(precision="fp64", convergence=1e-9, diis_tol=1e-6, log=lambda s: log_fn("scale fp64 " + s))
-    rmx = scf.run(precision="mixed", mixed_until=1e-5, incremental=True, rebuild_every=8,
-                  convergence=1e-9, diis_tol=1e-6, log=lambda s: log_fn("scale mixed " + s))
-    P = r64.P.contiguous()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    scf.jk(P, "full")
-    torch.cuda.synchronize()
-    t_build = time.perf_counter() - t0
-    st = scf.engine.stats()
-    return {"workload": f"(H2O)64 RHF/cc-pVTZ Cartesian, {mol.n_bf} basis functions, Schwarz 1e-11",
+    mol = water_cluster(32, "6-311G(d,p)", seed=SEED)
+    scf = GPUSCF(mol, schwarz=1e-9, fp32_threshold=FP32_THR)
+    P0 = scf.guess()
+    scf.jk(P0, "full"), scf.jk(P0, "full_mixed")
+    r64 = scf.run(precision="fp64", convergence=1e-9, diis_tol=1e-6)
+    rmx = scf.run(precision="mixed", convergence=1e-9, diis_tol=1e-6)
+    J, K = scf.jk(r64.P, "full")
+    Jm, Km = scf.jk(r64.P, "full_mixed")
+    dF = float(torch.max(torch.abs((Jm - 0.5 * Km) - (J - 0.5 * K))).item())
+    log_fn(f"(H2O)32 6-311G(d,p): FP64 SCF {r64.wall:.2f}s, mixed {rmx.wall:.2f}s, "
+           f"dE {rmx.energy - r64.energy:+.1e}, mixed build dF {dF:.1e}")
+    return {"workload": f"(H2O)32 RHF/6-311G(d,p) Cartesian, {mol.n_bf} basis functions, Schwarz 1e-9",
             "scf_wall_fp64_s": r64.wall, "scf_wall_mixed_s": rmx.wall,
+            "scf_speedup_mixed": r64.wall / rmx.wall, "dE_mixed_vs_fp64": rmx.energy - r64.energy,
             "scf_iterations_fp64": r64.iterations, "scf_iterations_mixed": rmx.iterations,
-            "scf_builds_mixed": rmx.n_builds, "build_seconds_fp64": r64.build_seconds,
-            "build_seconds_mixed": rmx.build_seconds,
-            "delta_cutoffs": [scf.delta_cutoff, scf.delta_density_cutoff], "loose_cutoff": scf.loose_cutoff,
-            "mixed_delta_cutoffs": [scf.mixed_delta_cutoff, scf.mixed_delta_density_cutoff],
-            "energy_fp64": r64.energy, "dE_mixed_vs_fp64": rmx.energy - r64.energy,
-            "fp64_build_s_at_converged_density": t_build, "quartets_per_build": st["quartets"],
-            "prim_quartets_per_build": st["prim_quartets"]}
+            "mixed_build_max_dF_at_converged_density": dF}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--waters", type=int, default=N_WATER, help="cluster size (default 64)")
     ap.add_argument("--density", default=None,
                     help="converged density (.npy) to time the steps at; skips both SCF runs")
     ap.add_argument("--save-density", default=None)
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-molecules leg")
+    ap.add_argument("--profile-iterations", type=int, default=0,
+                    help="profiling runs: skip both timed SCFs, time the steps at the density "
+                         "of this many FP64 SCF iterations from the promolecule guess")
+    ap.add_argument("--no-side", action="store_true", help="skip the side legs (configs 2-4)")
     ap.add_argument("--guess-clusters", type=int, default=10,
                     help="held-out clusters of the learned-guess leg (0: skip)")
-    ap.add_argument("--scale", action="store_true",
-                    help="also run the (H2O)64 / cc-pVTZ SCF leg (several minutes)")
-    ap.add_argument("--delta-scale", type=float, default=None,
-                    help="cutoff factor of the FP64 incremental builds in the scale leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM traffic "
                          "calibration for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE) and exit")
@@ -235,17 +247,31 @@ def main():
         print(json.dumps({"copy_mib": a.copy_calibration_mib, "dispatches": 3,
                           "bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8}))
         return
-    mol = water_cluster(N_WATER, BASIS, seed=SEED)
-    log(f"workload: (H2O){N_WATER} {BASIS} n_bf={mol.n_bf} n_shell={mol.n_shell}")
+    mol = water_cluster(a.waters, BASIS, seed=SEED)
+    log(f"workload: (H2O){a.waters} {BASIS} n_bf={mol.n_bf} n_shell={mol.n_shell}")
     scf = GPUSCF(mol, schwarz=SCHWARZ, fp32_threshold=FP32_THR)
     log(f"setup (1e ints + Schwarz) {scf.setup_s:.2f}s")
     extra = {}
-    if a.density is None:
+    eng = scf.engine
+    if a.profile_iterations:
+        from tonto_amd.scf import run_rhf
+        r = run_rhf(scf.h, scf.S, mol.n_electrons, mol.nuclear_repulsion(), scf.jk, P0=scf.guess(),
+                    max_iterations=a.profile_iterations, log=lambda s: log("profile " + s))
+        P = r.P.contiguous()
+    elif a.density is None:
+        # warm-up for both runs alike: atomic densities cached, every kernel loaded
+        t0 = time.perf_counter()
+        P0 = scf.guess()
+        scf.jk_scaled(P0, "full", 1e3, 1.0)
+        scf.jk_scaled(P0, "full_mixed", 1e3, 1.0)
+        torch.cuda.synchronize()
+        log(f"warm-up {time.perf_counter() - t0:.1f}s")
         r64 = scf.run(precision="fp64", convergence=1e-9, diis_tol=1e-6,
                       log=lambda s: log("fp64 " + s))
-        rmx = scf.run(precision="mixed", mixed_until=1e-5, incremental=True, rebuild_every=8,
-                      convergence=1e-9, diis_tol=1e-6, log=lambda s: log("mixed " + s))
+        rmx = scf.run(precision="mixed", convergence=1e-9, diis_tol=1e-6,
+                      log=lambda s: log("mixed " + s))
         dE = rmx.energy - r64.energy
+        n64 = r64.n_builds["full"]
         extra.update({
             "scf_energy_fp64": r64.energy, "scf_energy_mixed": rmx.energy,
             "dE_mixed_vs_fp64": dE, "dE_within_1e-8": bool(abs(dE) <= 1e-8),
@@ -254,16 +280,21 @@ def main():
             "scf_iterations_fp64": r64.iterations, "scf_iterations_mixed": rmx.iterations,
             "scf_builds_mixed": rmx.n_builds, "build_seconds_mixed": rmx.build_seconds,
             "build_seconds_fp64": r64.build_seconds,
-            "scf_converged": bool(r64.converged and rmx.converged)})
+            "fp64_build_s_mean_in_scf": r64.build_seconds["full"] / max(1, n64),
+            "fp64_build_s_last_in_scf": r64.history[-1]["build_seconds"],
+            "scf_converged": bool(r64.converged and rmx.converged),
+            "mixed_schedule": {"loose_cutoff": scf.loose_cutoff, "switch_diis_error": scf.switch_error,
+                               "mixed_delta_cutoffs": [scf.mixed_delta_cutoff, scf.mixed_delta_density_cutoff],
+                               "fp64_delta_cutoffs": [scf.delta_cutoff, scf.delta_density_cutoff]}})
         log(f"SCF fp64 E={r64.energy:.10f} {r64.wall:.2f}s it={r64.iterations}; "
-            f"mixed E={rmx.energy:.10f} {rmx.wall:.2f}s it={rmx.iterations} dE={dE:.2e}")
+            f"mixed E={rmx.energy:.10f} {rmx.wall:.2f}s it={rmx.iterations} dE={dE:.2e} "
+            f"speedup {r64.wall / rmx.wall:.2f}")
         P = r64.P.contiguous()
         if a.save_density:
             np.save(a.save_density, P.cpu().numpy())
     else:
         P = torch.tensor(np.load(a.density), device="cuda")
 
-    eng = scf.engine
     eng.set_profiling(True)
     ncls = len(class_pairs())
 
@@ -278,76 +309,73 @@ def main():
             times += eng.class_times()        # HIP events of this step (synchronises)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        st = eng.stats()
-        cnt = eng.class_counters()
-        return wall, st, cnt, times / steps
+        return wall, eng.stats(), eng.class_counters(), times / max(1, steps)
 
-    wall64, st64, cnt64, t64 = timed("full", a.steps, a.warmup)
     wallmx, stmx, cntmx, tmx = timed("full_mixed", a.steps, a.warmup)
+    st64 = None
+    if a.density is not None or a.profile_iterations:
+        # no FP64 SCF ran: one FP64 build for the comparison
+        _, st64, cnt64, t64 = timed("full", 1, 0)
     q = stmx["quartets"]
     value = q * a.steps / wallmx
-    f64, p64 = build_flops(cnt64)
     fmx, pmx = build_flops(cntmx)
     for kk, cp in enumerate(class_pairs()):
-        if cnt64[kk][0] or cntmx[kk][0]:
-            log("class (%d%d|%d%d) quartets %10d prim %11d | fp64 %8.3f ms %6.2f TF | mixed %8.3f ms %6.2f TF fp32 %d"
-                % (*cp, cnt64[kk][0], cnt64[kk][1], t64[kk], p64[kk] / max(t64[kk], 1e-9) / 1e9,
-                   tmx[kk], pmx[kk] / max(tmx[kk], 1e-9) / 1e9, cntmx[kk][2]))
-    log(f"build fp64 {wall64 / a.steps * 1e3:.1f} ms ({f64 / (wall64 / a.steps) / 1e12:.2f} TF), "
-        f"mixed {wallmx / a.steps * 1e3:.1f} ms ({fmx / (wallmx / a.steps) / 1e12:.2f} TF)")
-    flops, per_cls = build_flops(cntmx)
+        if cntmx[kk][0]:
+            log("class (%d%d|%d%d) quartets %11d prim %12d fp32 %11d | mixed %9.3f ms %6.2f TF"
+                % (*cp, cntmx[kk][0], cntmx[kk][1], cntmx[kk][2], tmx[kk], pmx[kk] / max(tmx[kk], 1e-9) / 1e9))
+    ms_step = wallmx / a.steps * 1e3
+    log(f"mixed build {ms_step:.1f} ms ({fmx / (ms_step * 1e-3) / 1e12:.2f} TF algorithmic)")
     k = int(np.argmax(tmx))
     la, lb, lc, ld = class_pairs()[k]
-    achieved = per_cls[k] / (tmx[k] * 1e-3) / 1e12
-    flops64, per64 = build_flops(cnt64)
-    k64 = int(np.argmax(t64))
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    achieved = pmx[k] / (tmx[k] * 1e-3) / 1e12
+    forms = eng.variants()
+    roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "traffic_read_written": None,
-            "kernel": f"jk class ({la}{lb}|{lc}{ld}) mixed",
-            "kernel_ms": float(tmx[k]), "kernel_flops": per_cls[k],
+            "kernel": f"jk class ({la}{lb}|{lc}{ld}) mixed, {'wavefront' if forms[k] else 'lane'} form",
+            "kernel_ms_per_build": float(tmx[k]), "kernel_flops_per_build": pmx[k],
             "fp32_quartet_share": float(cntmx[k][2]) / max(1.0, float(cntmx[k][0])),
-            "note": "FP64 vector-ALU bound path (no MFMA); MI355X's FP64 vector and FP64 matrix "
-                    "peaks are both 78.6 TFLOP/s; FLOPs by tonto_amd/flops.py",
-            "fp64_path": {"kernel": "jk class (%d%d|%d%d) fp64" % class_pairs()[k64],
-                          "kernel_ms": float(t64[k64]),
-                          "achieved": per64[k64] / (t64[k64] * 1e-3) / 1e12,
-                          "build_tflops": flops64 / (wall64 / a.steps) / 1e12}}
-    tr = TRAFFIC.get(roof["kernel"])
-    if tr is not None and mol.n_bf == 992:
+            "build_tflops": fmx / (ms_step * 1e-3) / 1e12,
+            "build_frac": fmx / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "note": "FP64 vector-ALU bound path (Rys recurrences, (e0|f0) assembly and transfer are "
+                    "FMA chains; no MFMA); MI355X FP64 vector peak 78.6 TFLOP/s; FP32 quartets counted "
+                    "against the FP64 peak (non-packed FP32 issues at the FP64 rate); FLOPs by "
+                    "tonto_amd/flops.py; kernel time = HIP events around the class pair's launches on "
+                    "the library's stream"}
+    tr = TRAFFIC.get(roof["kernel"].split(",")[0])
+    if tr is not None and mol.n_bf == 4160:
         roof["traffic"] = tr["bytes"]
         roof["traffic_read_written"] = [tr["read"], tr["written"]]
-        roof["traffic_note"] = ("HBM bytes per build of this kernel; the whole build's algorithmic "
-                                "bytes are ~24 MB (P read, X and Y written): the J/K atomic adds and "
-                                "the K-row flushes are performed at the memory side, the density "
-                                "gathers miss the XCD's L2")
+    cfg = {"workload": f"(H2O){a.waters} RHF/{BASIS} Cartesian (f on O), {mol.n_bf} basis functions, "
+                       f"seeded cubic-lattice cluster, Schwarz {SCHWARZ:g}; one step = one J/K build at the "
+                       f"converged density, FP32 below Schwarz bound {FP32_THR:g}",
+           "n_bf": mol.n_bf, "schwarz_cutoff": SCHWARZ,
+           "quartets_per_step": q, "prim_quartets_per_step": stmx["prim_quartets"],
+           "fp32_quartets_per_step": stmx["fp32_quartets"],
+           "build_tflops_mixed": fmx / (ms_step * 1e-3) / 1e12, **extra}
+    if "fp64_build_s_last_in_scf" in extra:
+        # the FP64 SCF's last full build ran at the converged density: same quartet set
+        cfg["fp64_quartets_per_s"] = q / extra["fp64_build_s_last_in_scf"]
+    if st64 is not None:
+        cfg["fp64_build_ms"] = float(np.sum(t64))
+        cfg["fp64_quartets_per_s"] = st64["quartets"] / (np.sum(t64) * 1e-3)
     out = {"metric": METRIC, "value": value, "unit": "quartets/s", "n_gpus": 1,
-           "steps": a.steps, "warmup": a.warmup, "ms_per_step": wallmx / a.steps * 1e3,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "fp64+fp32", "data": "synthetic",
-           "config": {"workload": f"(H2O){N_WATER} RHF/{BASIS} Cartesian, {mol.n_bf} basis "
-                                  f"functions, seeded cubic-lattice cluster; one step = one J/K "
-                                  f"build at the converged density, FP32 below Schwarz bound "
-                                  f"{FP32_THR:g}",
-                      "n_bf": mol.n_bf, "schwarz_cutoff": SCHWARZ,
-                      "quartets_per_step": q, "prim_quartets_per_step": stmx["prim_quartets"],
-                      "fp32_quartets_per_step": stmx["fp32_quartets"],
-                      "fp64_quartets_per_s": st64["quartets"] * a.steps / wall64,
-                      "fp64_ms_per_step": wall64 / a.steps * 1e3,
-                      "build_tflops_mixed": flops / (wallmx / a.steps) / 1e12,
-                      **extra},
-           "roofline": roof}
-    if not a.no_batch and a.density is None:
+           "config": cfg, "roofline": roof}
+    if not a.no_side and a.density is None and not a.profile_iterations:
+        out["config"]["config2_h2o32_6311gdp"] = config2_leg(log)
         out["config"]["batched_builds"] = batch_leg(a.steps, log)
-    if a.guess_clusters > 0 and a.density is None:
-        out["config"]["learned_guess"] = guess_leg(a.guess_clusters, log)
-    if a.scale:
-        out["config"]["scale_h2o64_cc_pvtz"] = scale_leg(log, a.delta_scale)
+        if a.guess_clusters > 0:
+            out["config"]["learned_guess"] = guess_leg(a.guess_clusters, log)
     if not a.no_cpu:
-        cb = cpu_baseline(mol, P.cpu().numpy(), a.cpu_seconds)
-        log(f"cpu baseline {cb['value']:.3e} quartets/s on {cb['cores']} threads ({cb['seconds']:.1f}s)")
+        cb = cpu_baseline(mol, P.cpu().numpy(), scf.max_I, a.cpu_seconds)
+        log(f"cpu baseline {cb['value']:.3e} quartets/s on {cb['cores']} threads, "
+            f"{cb['one_thread']['value']:.3e} on one ({cb['seconds']:.1f}s)")
         cb.pop("seconds")
         out["cpu_baseline"] = cb
+        if out["value"]:
+            out["config"]["gpu_over_cpu_quartets_per_s"] = out["value"] / cb["value"]
     print(json.dumps(out), flush=True)
 
 

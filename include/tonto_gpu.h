@@ -105,6 +105,18 @@ int tgpu_set_stream(tgpu_ctx *ctx, void *hip_stream);
 int tgpu_set_cutoffs(tgpu_ctx *ctx, double schwarz, double j_density, double k_density,
                      double pair);
 
+/* Spherical basis (Tonto's use_spherical_basis= TRUE, one molecule per context): with
+ * on != 0 the matrices of tgpu_make_r_JK[_device] and tgpu_one_electron are n_sph x n_sph
+ * (real solid harmonics in Molden order M = 0, +1, -1, ... for d and f shells, s and p
+ * Cartesian; tgpu_dims reports n_sph), and the build is make_r_Fock_mx's spherical branch
+ * MOLECULE.FOCK:make_r_JK_direct (molecule.fock.foo:123-128, 1235-1363): max_I from the
+ * solid-harmonic (ab|ab) blocks, P_max over solid-harmonic blocks, Schwarz_inequality_test
+ * (3454-3495: the largest of the six density blocks, one decision for J and K), integrals
+ * as by SHELL1QUARTET:change_to_spherical (shell1quartet.foo:505-563).  Call before
+ * tgpu_schwarz (a later switch redoes the Schwarz table).  UHF, J-only/K-only and the
+ * allocation-free entry stay Cartesian. */
+int tgpu_set_spherical(tgpu_ctx *ctx, int on);
+
 /* Mixed-precision threshold: with TGPU_JK_MIXED, quartets with
  * max_I(ab)*max_I(cd) < threshold run in FP32 (default 1e-6). */
 int tgpu_set_fp32_threshold(tgpu_ctx *ctx, double threshold);

@@ -143,3 +143,34 @@ def test_config5_mixed_scf_cc_pvtz(torch_cuda):
           f"({rmx.iterations} it, builds {rmx.n_builds})")
     assert r64.converged and rmx.converged
     assert abs(rmx.energy - r64.energy) <= 1e-8
+
+
+def test_uhf_scf_cation_reaches_g09(torch_cuda):
+    """H2O+ UHF/cc-pVDZ (6D) on the GPU path (make_u_JK_engine builds on the device, the
+    UHF loop in torch): g09 -75.6327038760 (scripts/check_dft_reference.py:72)."""
+    from tonto_amd.driver import GPUSCF
+    from tonto_amd.geometry import tonto_water
+    r = GPUSCF(tonto_water("cc-pVDZ", charge=1)).run_uhf(multiplicity=2, convergence=1e-10, diis_tol=1e-7)
+    print(f"GPU UHF H2O+ E = {r.energy:.10f} ({r.iterations} iterations)")
+    assert r.converged and abs(r.energy - (-75.6327038760)) < 1e-9
+
+
+def test_uhf_scf_cluster_against_oracle(torch_cuda, oracle_mod):
+    """(H2O)2+ UHF/6-31G(d) at Schwarz 1e-10: the GPU UHF SCF's energy against the same
+    SCF driven by the oracle's UHF J/K (quartets over four distinct atoms, skip lists live)."""
+    import torch
+    from tonto_amd.driver import GPUSCF
+    from tonto_amd.geometry import water_cluster
+    from tonto_amd.scf import run_uhf
+    mol = water_cluster(2, "6-31G(d)", seed=3)
+    mol.charge = 1
+    scf = GPUSCF(mol, schwarz=1e-10)
+    r = scf.run_uhf(multiplicity=2, guess="core", convergence=1e-10, diis_tol=1e-7)
+    o = oracle_mod.OracleSystem(mol)
+
+    def jk(Pa, Pb):
+        return tuple(torch.from_numpy(x) for x in o.make_u_JK(Pa.numpy(), Pb.numpy(), schwarz=1e-10,
+                                                             max_I=scf.max_I))
+    ro = run_uhf(scf.h.cpu(), scf.S.cpu(), 10, 9, mol.nuclear_repulsion(), jk, convergence=1e-10, diis_tol=1e-7)
+    print(f"(H2O)2+ UHF: GPU {r.energy:.10f} oracle {ro.energy:.10f}")
+    assert r.converged and ro.converged and abs(r.energy - ro.energy) < 1e-9

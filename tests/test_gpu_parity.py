@@ -365,3 +365,40 @@ def test_uhf_mixed_precision(torch_cuda, oracle_mod):
     dFb = np.max(np.abs((Jam + Jbm - Kbm) - (Ja + Jb - Kb)))
     print("UHF mixed dF", dFa, dFb, qmx)
     assert dFa < 1e-7 and dFb < 1e-7
+
+
+@pytest.mark.parametrize("nw,basis,cut", [(1, "cc-pVTZ", 1e-9), (2, "cc-pVTZ", 1e-11), (3, "6-31G(d)", 1e-10)])
+def test_spherical_direct_build_against_oracle(torch_cuda, oracle_mod, nw, basis, cut):
+    """Spherical basis (make_r_JK_direct): the GPU's Cartesian kernels on T P T^T with
+    J, K turned back into solid harmonics, against the oracle's per-quartet solid-harmonic
+    integrals digested with the spherical density; spherical Schwarz table to 1e-12
+    relative (the two sum the solid-harmonic transform in different orders), the same
+    quartets (Schwarz_inequality_test, both on the GPU's table), J and K to 1e-10."""
+    from tonto_amd.geometry import tonto_water, water_cluster
+    mol = tonto_water(basis) if nw == 1 else water_cluster(nw, basis, seed=9)
+    e = _engine(mol, schwarz=cut, spherical=True)
+    o = oracle_mod.OracleSystem(mol)
+    assert e.n_bf == o.n_sph
+    mi = e.schwarz()
+    mo = o.schwarz_sph()
+    assert np.max(np.abs(mi - mo) / np.maximum(mo, 1e-300)) < 1e-12
+    P = _rand_density(e.n_bf, 31, 0.5)
+    J, K = e.make_r_JK(P)
+    st = e.stats()
+    Jo, Ko, so = o.make_r_JK_direct(P, schwarz=cut, max_I=mi, return_stats=True)
+    dJ, dK = np.max(np.abs(J - Jo)), np.max(np.abs(K - Ko))
+    print(f"spherical {basis} n={e.n_bf}: dJ={dJ:.2e} dK={dK:.2e} quartets gpu={st['quartets']} oracle={so['quartets']}")
+    assert st["quartets"] == so["quartets"] and st["prim_quartets"] == so["prim_quartets"]
+    assert dJ <= 1e-10 and dK <= 1e-10
+
+
+def test_spherical_scf_cc_pvtz_water(torch_cuda):
+    """H2O RHF/cc-pVTZ in the spherical basis on the GPU path (58 functions): Tonto's
+    tests/short/h2o_rhf_cc-pVTZ_spherical_harmonic_basis/stdout:204 prints -76.0524."""
+    from tonto_amd.driver import GPUSCF
+    from tonto_amd.geometry import tonto_water
+    scf = GPUSCF(tonto_water("cc-pVTZ"), spherical=True)
+    assert scf.engine.n_bf == 58
+    r = scf.run(convergence=1e-9, diis_tol=1e-6)
+    print(f"spherical cc-pVTZ water: E = {r.energy:.10f}")
+    assert r.converged and round(r.energy, 4) == -76.0524

@@ -320,6 +320,17 @@ def main():
              " \\\n".join(f"    M({a}, {b}, {int(is_heavy(a, b, a, b))})" for (a, b) in pcs)]
     with open(os.path.join(gdir, "classes.inc"), "w") as f:
         f.write("\n".join(lines) + "\n")
+    # Cartesian -> real solid harmonic matrices of the spherical basis, l = 2 .. LMAX
+    from tonto_amd.sph import solid_harmonic_tables
+    T = solid_harmonic_tables(LMAX)
+    out = ["/* Generated by gen_eri.py -- do not edit. */",
+           "/* tgpu_sph_T<l>[c * (2l+1) + M]: Cartesian component c (Tonto order) of the real solid",
+           "   harmonic M (Molden order 0, +1, -1, ...), tonto_amd/sph.py */"]
+    for l in range(2, LMAX + 1):
+        vals = ", ".join(repr(float(v)) for v in T[l].ravel())
+        out.append(f"static __device__ const double tgpu_sph_T{l}[{T[l].size}] = {{{vals}}};")
+    with open(os.path.join(gdir, "sph.inc"), "w") as f:
+        f.write("\n".join(out) + "\n")
     print(f"wrote {gdir}: {len(qcs)} classes, heavy: "
           + " ".join(f"({a}{b}|{c}{d})" for (a, b, c, d) in qcs if is_heavy(a, b, c, d)))
 

@@ -65,7 +65,6 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
     }
 #endif
     (void)variant;
-    (void)mixed;  // the wavefront form runs in FP64
     using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS>;
     // work-group (x, y): bra pairs x, x + gx, ...; of each, ket chunks y, y + gy, ... of
     // a.ket_chunk kets.  gy splits long prefixes when there are few bra pairs, so that the
@@ -87,7 +86,11 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
     gy = std::min<long long>(gy, 65535);
     JKArgs b = a;
     b.ket_chunk = kc;
-    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS><<<dim3((unsigned)gx, (unsigned)gy), 64, 0, s>>>(b);
+    // FP64 over the kets before the FP32 split of each prefix (all of it in an FP64 build),
+    // then FP32 over the rest
+    if (!mixed) b.fp32_thr = 0.0;
+    jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double><<<dim3((unsigned)gx, (unsigned)gy), 64, 0, s>>>(b);
+    if (mixed) jk_wave_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, float><<<dim3((unsigned)gx, (unsigned)gy), 64, 0, s>>>(b);
     return 0;
 }
 
@@ -98,17 +101,17 @@ size_t TGPU_WS_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)()
 
 #if TGPU_LP == TGPU_LR && TGPU_LQ == TGPU_LS
 void TGPU_SCHWARZ_NAME_X(TGPU_LP, TGPU_LQ)(const DevPairs &pr, const DevPrims &pm, int start, int n,
-                                          double *maxI, double *ws, long long ws_doubles, hipStream_t s)
+                                          double *maxI, double *ws, long long ws_doubles, int sph, hipStream_t s)
 {
 #if TGPU_HEAVY
     using C = WaveCfg<TGPU_LP, TGPU_LQ, TGPU_LP, TGPU_LQ>;
     const long long slots = ws_slot_count(ws_doubles, C::WS);
     const int grid = C::GB ? (int)(n < slots ? n : slots) : n;
-    if (grid > 0) schwarz_wave_kernel<TGPU_LP, TGPU_LQ><<<grid, 64, 0, s>>>(pr, pm, start, n, maxI, ws);
+    if (grid > 0) schwarz_wave_kernel<TGPU_LP, TGPU_LQ><<<grid, 64, 0, s>>>(pr, pm, start, n, maxI, ws, sph);
 #else
     (void)ws;
     (void)ws_doubles;
-    schwarz_kernel<TGPU_LP, TGPU_LQ><<<(n + 63) / 64, 64, 0, s>>>(pr, pm, start, n, maxI);
+    schwarz_kernel<TGPU_LP, TGPU_LQ><<<(n + 63) / 64, 64, 0, s>>>(pr, pm, start, n, maxI, sph);
 #endif
 }
 #endif

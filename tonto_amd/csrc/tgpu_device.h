@@ -5,6 +5,7 @@
 
 #include "../../include/tonto_gpu.h"
 #include "gen/classes.inc"
+#include "gen/sph.inc"
 
 #define RYS_QUAL static __device__ const __attribute__((aligned(16)))
 #include "rys_table.inc"
@@ -14,7 +15,7 @@
 // Rys coefficient loads of the lane-per-quartet kernels: batches of about this many
 // (root, weight) pairs issued together (0: one step of the recurrence at a time)
 #ifndef TGPU_RYS_BATCH
-#define TGPU_RYS_BATCH 0
+#define TGPU_RYS_BATCH 7
 #endif
 
 // ----------------------------------------------------------------------------------
@@ -230,6 +231,39 @@ static __device__ void rys_roots_rt(int n, double X, double *u, double *w)
 
 static constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 static constexpr int pair_class(int lp, int lq) { return lp * (lp + 1) / 2 + lq; }
+// functions of an l-shell in Tonto's spherical basis: real solid harmonics for l >= 2,
+// the Cartesian functions for s and p (shell1quartet.foo:505-563)
+__host__ __device__ constexpr int nsph(int l) { return l >= 2 ? 2 * l + 1 : ncart(l); }
+
+// coefficient of Cartesian component c (Tonto order) in solid harmonic m of an l-shell
+__device__ __forceinline__ double sph_t(int l, int c, int m)
+{
+    return l == 2 ? tgpu_sph_T2[c * 5 + m] : l == 3 ? tgpu_sph_T3[c * 7 + m] : (c == m ? 1.0 : 0.0);
+}
+
+// (ab|ab) in the spherical basis for solid harmonics (alpha of shell P, beta of shell Q)
+// from the Cartesian block I[((p NQ + q) NP + p') NQ + q'] of the pair (P Q|P Q)
+template <int LP, int LQ>
+__device__ __forceinline__ double sph_abab(const double *I, int alpha, int beta)
+{
+    constexpr int NP = ncart(LP), NQ = ncart(LQ);
+    double v = 0.0;
+    for (int p1 = 0; p1 < NP; ++p1) {
+        const double t1 = sph_t(LP, p1, alpha);
+        if (t1 == 0.0) continue;
+        for (int q1 = 0; q1 < NQ; ++q1) {
+            const double t2 = t1 * sph_t(LQ, q1, beta);
+            if (t2 == 0.0) continue;
+            for (int p2 = 0; p2 < NP; ++p2) {
+                const double t3 = t2 * sph_t(LP, p2, alpha);
+                if (t3 == 0.0) continue;
+                for (int q2 = 0; q2 < NQ; ++q2)
+                    v += t3 * sph_t(LQ, q2, beta) * I[((p1 * NQ + q1) * NP + p2) * NQ + q2];
+            }
+        }
+    }
+    return v;
+}
 
 template <int LP, int LQ, int LR, int LS, typename R> struct EriBody;
 template <int LP, int LQ, int LR, int LS> struct WaveTab;
@@ -306,6 +340,9 @@ struct JKArgs {
     // larger Tonto index, for ab != cd (the updates Tonto's UHF engine puts into JA
     // through JBcd => JA(cd), molecule.fock.foo:2637)
     int j_ket_nonsame;
+    // spherical basis: Tonto's make_r_JK_direct screening (Schwarz_inequality_test), one
+    // decision for J and K
+    int sph_screen;
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
@@ -381,8 +418,16 @@ __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int 
     const double JP = bhi ? fmax(b.Pmax_b, Pmax_k) : fmax(Pmax_k, b.Pmax_b);
     const double KP = fmax(fmax(ldc(b.Pmax + tonto_pair(b.bp, kr)), ldc(b.Pmax + tonto_pair(b.bp, ks))),
                            fmax(ldc(b.Pmax + tonto_pair(b.bq, kr)), ldc(b.Pmax + tonto_pair(b.bq, ks))));
-    t.skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
-    t.skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
+    if (a.sph_screen) {
+        // Schwarz_inequality_test (molecule.fock.foo:3454-3495): the largest of the six
+        // density blocks, skip below TOL(20) or unless Pmax*I_max(ab)*I_max(cd) > cutoff
+        const double Pm = fmax(JP, KP);
+        const bool skip = (Pm < 1e-20) || !(Pm * Iab * Icd > a.cutoff);
+        t.skipJ = t.skipK = skip;
+    } else {
+        t.skipJ = (JP < a.jp) || (JP * Iab * Icd < a.cutoff);
+        t.skipK = (KP < a.kp) || (KP * Iab * Icd < a.cutoff);
+    }
     if (t.skipJ && t.skipK) return false;
     t.halvings = (b.bp == b.bq) + (kr == ks) + (b.sb == t.sk);
     t.f = t.halvings == 0 ? 1.0 : t.halvings == 1 ? 0.5 : t.halvings == 2 ? 0.25 : 0.125;
@@ -691,7 +736,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 // (molecule.fock.foo:3712-3752).  One thread per shell pair of one class bin.
 template <int LP, int LQ>
 __global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, int start, int n,
-                                                    double *maxI)
+                                                    double *maxI, int sph)
 {
     constexpr int NP = ncart(LP), NQ = ncart(LQ);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -701,10 +746,16 @@ __global__ __launch_bounds__(64) void schwarz_kernel(DevPairs pr, DevPrims pm, i
     double I[NP * NQ * NP * NQ];
     EriBody<LP, LQ, LP, LQ, double>::run(pm, v, v, I, rys_table_data + rys_table_offset[EriBody<LP, LQ, LP, LQ, double>::NROOT]);
     double m = 0.0;
+    if (sph && (LP >= 2 || LQ >= 2)) {
+        // spherical basis: the largest diagonal element of the solid-harmonic block
+        for (int al = 0; al < nsph(LP); ++al)
+            for (int be = 0; be < nsph(LQ); ++be) m = fmax(m, fabs(sph_abab<LP, LQ>(I, al, be)));
+    } else {
 #pragma unroll
-    for (int p = 0; p < NP; ++p)
+        for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) m = fmax(m, fabs(I[((p * NQ + q) * NP + p) * NQ + q]));
+            for (int q = 0; q < NQ; ++q) m = fmax(m, fabs(I[((p * NQ + q) * NP + p) * NQ + q]));
+    }
     maxI[s] = sqrt(m);
 }
 
@@ -727,209 +778,359 @@ __host__ __device__ constexpr long long ws_slot_count(long long ws_doubles, long
 // grid), so that the 160 KiB of a CU still holds several work-groups.
 #define TGPU_WAVE_LDS_BUDGET (64 * 1024)
 
+// (e0|f0) entries a lane of the wavefront form accumulates in registers over the
+// primitive quartets; with more, the accumulation goes through the transfer buffer
+#ifndef TGPU_WAVE_GREG_MAX
+#define TGPU_WAVE_GREG_MAX 4
+#endif
+// LDS of one wavefront's teams together in the wavefront form (see WaveCfg::TEAMS)
+#ifndef TGPU_WAVE_TEAM_LDS
+#define TGPU_WAVE_TEAM_LDS (16 * 1024)
+#endif
+
 template <int LP, int LQ, int LR, int LS>
 struct WaveCfg {
     using T = WaveTab<LP, LQ, LR, LS>;
-    static constexpr int NG = T::NE * T::NF, MG = (NG + 63) / 64;
+    static constexpr int NG = T::NE * T::NF;
     static constexpr int NGT = 3 * T::NROOT * (T::LPQ + 1) * (T::LRS + 1);
     static constexpr int NDEN = T::NP * T::NQ + T::NR * T::NS + T::NS * T::NQ + T::NR * T::NQ +
                                 T::NS * T::NP + T::NR * T::NP;
-    // (e0|f0) accumulators in registers (at most 16 per lane); else in the buffer
-    static constexpr bool GREG = MG <= 16;
     // transfer buffers in the global workspace
     static constexpr bool GB = (size_t)(NGT + 2 * T::BUF + NDEN) * 8 > TGPU_WAVE_LDS_BUDGET;
     static constexpr int LBUF = GB ? 1 : T::BUF;
     // doubles of workspace per slot (0: LDS only)
     static constexpr size_t WS = GB ? 2 * (size_t)T::BUF : 0;
+    // LDS bytes of one quartet
+    static constexpr size_t TEAM_BYTES = (size_t)(NGT + 2 * LBUF + NDEN) * 8;
+    // Quartets one wavefront works on at once: its 64 lanes form TEAMS teams of S lanes,
+    // each team on its own quartet (same bra pair, different kets) with its own LDS, all
+    // in lockstep.  4 teams when their LDS fits TGPU_WAVE_TEAM_LDS and a team's lanes
+    // cover the 3n (axis, root) tasks of the table phase, else 2, else 1.
+    static constexpr int TEAMS = GB ? 1
+                                 : (4 * TEAM_BYTES <= TGPU_WAVE_TEAM_LDS && 3 * T::NROOT <= 16) ? 4
+                                 : (2 * TEAM_BYTES <= TGPU_WAVE_TEAM_LDS && 3 * T::NROOT <= 32) ? 2
+                                                                                                : 1;
+    static constexpr int S = 64 / TEAMS;
+    // (e0|f0) entries per lane; in registers up to TGPU_WAVE_GREG_MAX, else in the buffer
+    static constexpr int MG = (NG + S - 1) / S;
+    static constexpr bool GREG = MG <= TGPU_WAVE_GREG_MAX;
 };
 
+// One team's 2-D tables and transfer buffers, in the arithmetic type R of its quartet
+// (double, or float for the small-bound quartets of a mixed build)
+template <int LP, int LQ, int LR, int LS, typename R>
+struct WaveBuf {
+    using C = WaveCfg<LP, LQ, LR, LS>;
+    R g[C::NGT];
+    R buf[2][C::LBUF];
+};
+
+// One team's LDS: its buffers (either precision in the same place) and the density blocks
+// of its quartet
 template <int LP, int LQ, int LR, int LS>
 struct WaveLDS {
     using T = WaveTab<LP, LQ, LR, LS>;
-    using C = WaveCfg<LP, LQ, LR, LS>;
-    double g[C::NGT];
-    double buf[2][C::LBUF];
+    union {
+        WaveBuf<LP, LQ, LR, LS, double> d;
+        WaveBuf<LP, LQ, LR, LS, float> f;
+    } u;
     double Ppq[T::NP * T::NQ], Prs[T::NR * T::NS];
     double Psq[T::NS * T::NQ], Prq[T::NR * T::NQ], Psp[T::NS * T::NP], Prp[T::NR * T::NP];
 };
 
-// Computes (pq|rs) of one contracted quartet; returns the buffer holding [pq][rs] (LDS,
-// or the work-group's workspace slot gbuf of 2*BUF doubles for the classes that keep
-// their transfer buffers in global memory).  Must be called by all 64 lanes of the
-// work-group.
-template <int LP, int LQ, int LR, int LS>
-__device__ const double *wave_eri(const DevPrims &pm, const PairView &bra, const PairView &ket,
-                                  WaveLDS<LP, LQ, LR, LS> &L, double *gbuf)
+template <int LP, int LQ, int LR, int LS, typename R>
+__device__ __forceinline__ WaveBuf<LP, LQ, LR, LS, R> &wave_buf(WaveLDS<LP, LQ, LR, LS> &L)
+{
+    if constexpr (sizeof(R) == sizeof(double)) return L.u.d;
+    else return L.u.f;
+}
+
+// Computes (pq|rs) of one contracted quartet per team; returns the team's buffer holding
+// [pq][rs] (LDS, or the work-group's workspace slot gbuf of 2*BUF values for the classes
+// that keep their transfer buffers in global memory).  R is the type of the 2-D tables,
+// the (e0|f0) accumulation and the transfer relations; roots, weights and the recurrence
+// coefficients are computed in FP64.  All 64 lanes of the work-group call it together (it
+// synchronises); tl is the lane's index in its team of S lanes, B the team's buffers.  A
+// team with live == false (no quartet for it this round) follows along without
+// contributing; npmax, the largest primitive-quartet count over the live teams, is
+// uniform so that the loops and barriers are.
+template <int LP, int LQ, int LR, int LS, int S, typename R>
+__device__ const R *wave_eri(const DevPrims &pm, const PairView &bra, const PairView &ket,
+                             WaveBuf<LP, LQ, LR, LS, R> &B, R *gbuf, int tl, bool live, int npmax)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
     using C = WaveCfg<LP, LQ, LR, LS>;
     constexpr int N = T::NROOT, W1 = T::LPQ + 1, W2 = T::LRS + 1, TW = W1 * W2;
-    constexpr int NG = C::NG, MG = C::GREG ? C::MG : 1;
+    constexpr int NG = C::NG, MG = C::GREG ? (NG + S - 1) / S : 1;
     constexpr int NRS = T::NR * T::NS;
-    const int lane = threadIdx.x;
-    double *src, *dst;
+    R *src, *dst;
     if constexpr (C::GB) {
         src = gbuf;
         dst = gbuf + T::BUF;
     } else {
-        src = L.buf[0];
-        dst = L.buf[1];
+        src = B.buf[0];
+        dst = B.buf[1];
         (void)gbuf;
     }
-    double Gr[MG];
+    R Gr[MG];
     // (e0|f0) entry k reads the x, y, z tables at gidx[3k..3k+2] (< TW <= 64): packed 6 bits each
     int gi[MG];
     if constexpr (C::GREG) {
 #pragma unroll
         for (int m = 0; m < MG; ++m) {
-            Gr[m] = 0.0;
-            const int k = lane + 64 * m;
+            Gr[m] = (R)0;
+            const int k = tl + S * m;
             const int kk = k < NG ? k : 0;
             gi[m] = T::gidx[3 * kk] | (T::gidx[3 * kk + 1] << 6) | (T::gidx[3 * kk + 2] << 12);
         }
     } else {
-        // each lane owns entries lane, lane+64, ... until the transfer step
-        for (int k = lane; k < NG; k += 64) src[k] = 0.0;
+        // each lane owns entries tl, tl+S, ... until the transfer step
+        for (int k = tl; k < NG; k += S) src[k] = (R)0;
     }
-    const int ax = lane / N, rr = lane - (lane / N) * N;
-    for (int ip = 0; ip < bra.n; ++ip) {
-        const int ipp = bra.off + ip;
-        const PrimRec pb = ldc(pm.rec + ipp);
-        const double za = pb.zeta, Ka = pb.K, Px = pb.Px, Py = pb.Py, Pz = pb.Pz;
-        for (int iq = 0; iq < ket.n; ++iq) {
-            if (lane < 3 * N) {
-                const int iqq = ket.off + iq;
-                const PrimRec pk = ldc(pm.rec + iqq);
-                const double zc = pk.zeta, Kc = pk.K, Qx = pk.Px, Qy = pk.Py, Qz = pk.Pz;
-                const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
-                const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
-                const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
-                double ur, wr;
-                rys_root_one<N>(X, rr, ur, wr);
-                const double PA = (ax == 0 ? Px - bra.Ax : ax == 1 ? Py - bra.Ay : Pz - bra.Az);
-                const double QC = (ax == 0 ? Qx - ket.Ax : ax == 1 ? Qy - ket.Ay : Qz - ket.Az);
-                const double qp = QP[ax];
-                const double rz = ur * rho / za, re = ur * rho / zc;
-                const double b10 = (1.0 - rz) * 0.5 / za, b01 = (1.0 - re) * 0.5 / zc, b00 = ur * 0.5 * izs;
-                const double c00 = PA + rz * qp, d00 = QC - re * qp;
-                // the 2-D table I(e, f) column by column (two columns live), each column
-                // stored to LDS as soon as it is complete
-                double *dst = L.g + (ax * N + rr) * TW;
-                double cp[W1], cc[W1];
-                cc[0] = ax == 2 ? wr * TWOPI5ON2 * Ka * Kc * sqrt(rho) : 1.0;
+    const int ax = tl / N, rr = tl - (tl / N) * N;
+    const int np = live ? bra.n * ket.n : 0;
+    for (int ipq = 0; ipq < npmax; ++ipq) {
+        const bool on = ipq < np;
+        if (on && tl < 3 * N) {
+            const int ip = ipq / ket.n, iq = ipq - ip * ket.n;
+            const PrimRec pb = ldc(pm.rec + bra.off + ip);
+            const PrimRec pk = ldc(pm.rec + ket.off + iq);
+            const double za = pb.zeta, Ka = pb.K, Px = pb.Px, Py = pb.Py, Pz = pb.Pz;
+            const double zc = pk.zeta, Kc = pk.K, Qx = pk.Px, Qy = pk.Py, Qz = pk.Pz;
+            const double zs = za + zc, izs = 1.0 / zs, rho = za * zc * izs;
+            const double QP[3] = {Qx - Px, Qy - Py, Qz - Pz};
+            const double X = rho * (QP[0] * QP[0] + QP[1] * QP[1] + QP[2] * QP[2]);
+            double ur, wr;
+            rys_root_one<N>(X, rr, ur, wr);
+            const double PA = (ax == 0 ? Px - bra.Ax : ax == 1 ? Py - bra.Ay : Pz - bra.Az);
+            const double QC = (ax == 0 ? Qx - ket.Ax : ax == 1 ? Qy - ket.Ay : Qz - ket.Az);
+            const double qp = QP[ax];
+            const double rz = ur * rho / za, re = ur * rho / zc;
+            const R b10 = (R)((1.0 - rz) * 0.5 / za), b01 = (R)((1.0 - re) * 0.5 / zc), b00 = (R)(ur * 0.5 * izs);
+            const R c00 = (R)(PA + rz * qp), d00 = (R)(QC - re * qp);
+            // the 2-D table I(e, f) column by column (two columns live), each column
+            // stored to LDS as soon as it is complete
+            R *gt = B.g + (ax * N + rr) * TW;
+            R cp[W1], cc[W1];
+            cc[0] = ax == 2 ? (R)(wr * TWOPI5ON2 * Ka * Kc * sqrt(rho)) : (R)1;
 #pragma unroll
-                for (int e = 0; e + 1 < W1; ++e) cc[e + 1] = c00 * cc[e] + (e ? e * b10 * cc[e - 1] : 0.0);
+            for (int e = 0; e + 1 < W1; ++e) cc[e + 1] = c00 * cc[e] + (e ? (R)e * b10 * cc[e - 1] : (R)0);
 #pragma unroll
-                for (int e = 0; e < W1; ++e) dst[e * W2] = cc[e];
+            for (int e = 0; e < W1; ++e) gt[e * W2] = cc[e];
 #pragma unroll
-                for (int f = 0; f + 1 < W2; ++f) {
-                    double cn[W1];
+            for (int f = 0; f + 1 < W2; ++f) {
+                R cn[W1];
 #pragma unroll
-                    for (int e = 0; e < W1; ++e)
-                        cn[e] = d00 * cc[e] + (f ? f * b01 * cp[e] : 0.0) + (e ? e * b00 * cc[e - 1] : 0.0);
+                for (int e = 0; e < W1; ++e)
+                    cn[e] = d00 * cc[e] + (f ? (R)f * b01 * cp[e] : (R)0) + (e ? (R)e * b00 * cc[e - 1] : (R)0);
 #pragma unroll
-                    for (int e = 0; e < W1; ++e) {
-                        dst[e * W2 + f + 1] = cn[e];
-                        cp[e] = cc[e];
-                        cc[e] = cn[e];
-                    }
+                for (int e = 0; e < W1; ++e) {
+                    gt[e * W2 + f + 1] = cn[e];
+                    cp[e] = cc[e];
+                    cc[e] = cn[e];
                 }
             }
-            __syncthreads();
+        }
+        __syncthreads();
+        if (on) {
             if constexpr (C::GREG) {
 #pragma unroll
                 for (int m = 0; m < MG; ++m) {
                     const int i0 = gi[m] & 63, i1 = TW * N + ((gi[m] >> 6) & 63), i2 = 2 * TW * N + (gi[m] >> 12);
-                    double acc = 0.0;
+                    R acc = (R)0;
 #pragma unroll
-                    for (int r = 0; r < N; ++r) acc += L.g[i0 + r * TW] * L.g[i1 + r * TW] * L.g[i2 + r * TW];
+                    for (int r = 0; r < N; ++r) acc += B.g[i0 + r * TW] * B.g[i1 + r * TW] * B.g[i2 + r * TW];
                     Gr[m] += acc;
                 }
             } else {
-                for (int k = lane; k < NG; k += 64) {
+                for (int k = tl; k < NG; k += S) {
                     const int i0 = T::gidx[3 * k], i1 = TW * N + T::gidx[3 * k + 1],
                               i2 = 2 * TW * N + T::gidx[3 * k + 2];
-                    double acc = 0.0;
+                    R acc = (R)0;
 #pragma unroll
-                    for (int r = 0; r < N; ++r) acc += L.g[i0 + r * TW] * L.g[i1 + r * TW] * L.g[i2 + r * TW];
+                    for (int r = 0; r < N; ++r) acc += B.g[i0 + r * TW] * B.g[i1 + r * TW] * B.g[i2 + r * TW];
                     src[k] += acc;
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
     if constexpr (C::GREG) {
 #pragma unroll
         for (int m = 0; m < MG; ++m) {
-            const int k = lane + 64 * m;
+            const int k = tl + S * m;
             if (k < NG) src[k] = Gr[m];
         }
     }
     __syncthreads();
     // ket-side transfer, rows e (T::NE of them), entries per level
-    const double CD[3] = {ket.ABx, ket.ABy, ket.ABz};
+    const R CD[3] = {(R)ket.ABx, (R)ket.ABy, (R)ket.ABz};
     int nprev = T::NF;
     for (int lev = 0; lev < T::NCD; ++lev) {
         const int n = T::cd_n[lev], off = T::cd_off[lev];
-        for (int idx = lane; idx < T::NE * n; idx += 64) {
+        for (int idx = tl; idx < T::NE * n; idx += S) {
             const int e = idx / n, t = idx - e * n;
             dst[e * n + t] = src[e * nprev + T::cd_s1[off + t]] + CD[T::cd_ax[off + t]] * src[e * nprev + T::cd_s2[off + t]];
         }
         __syncthreads();
-        double *tmp = src; src = dst; dst = tmp;
+        R *tmp = src; src = dst; dst = tmp;
         nprev = n;
     }
     // bra-side transfer on columns rs
-    const double AB[3] = {bra.ABx, bra.ABy, bra.ABz};
+    const R AB[3] = {(R)bra.ABx, (R)bra.ABy, (R)bra.ABz};
     for (int lev = 0; lev < T::NAB; ++lev) {
         const int n = T::ab_n[lev], off = T::ab_off[lev];
-        for (int idx = lane; idx < n * NRS; idx += 64) {
+        for (int idx = tl; idx < n * NRS; idx += S) {
             const int t = idx / NRS, c = idx - t * NRS;
             dst[t * NRS + c] = src[T::ab_s1[off + t] * NRS + c] + AB[T::ab_ax[off + t]] * src[T::ab_s2[off + t] * NRS + c];
         }
         __syncthreads();
-        double *tmp = src; src = dst; dst = tmp;
+        R *tmp = src; src = dst; dst = tmp;
     }
     return src;
 }
 
-// The wavefront form's work for bra pair bi: the chunks blockIdx.y, blockIdx.y + gridDim.y,
-// ... of a.ket_chunk kets of its Schwarz prefix (only chunks that hold kets are visited).
-template <int LP, int LQ, int LR, int LS>
-__device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, double *gbuf)
+// index of the (k+1)-th set bit of m (64 if there is none)
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int k)
+{
+    for (int i = 0; i < k && m; ++i) m &= m - 1;
+    return m ? __builtin_ctzll(m) : 64;
+}
+
+// J and K digestion of one quartet's integrals I ([pq][rs], type R) by the S lanes of its
+// team: one atomic per target element; the bra-pair J block accumulates in Jpq
+template <int LP, int LQ, int LR, int LS, int S, typename R>
+__device__ __forceinline__ void wave_digest(const JKArgs &a, const BraInfo &b, const KetTest &t, int fr, int fs,
+                                            const R *__restrict__ I, const WaveLDS<LP, LQ, LR, LS> &L, int tl,
+                                            double *Jpq)
 {
     using T = WaveTab<LP, LQ, LR, LS>;
     constexpr int NP = T::NP, NQ = T::NQ, NR = T::NR, NS = T::NS;
-    constexpr int NPQ = NP * NQ, NRS = NR * NS, MJ = (NPQ + 63) / 64;
-    const int lane = threadIdx.x;
+    constexpr int NPQ = NP * NQ, NRS = NR * NS, MJ = (NPQ + S - 1) / S;
+    const int ld = b.ld, fp = b.fp, fq = b.fq;
+    const double f = t.f;
+    // J sides as in the lane form (both, or the UHF pass's Tonto ket side only)
+    bool j_ket = true, j_bra = true;
+    if (a.j_ket_nonsame) {
+        const int tk = ldc(a.pr.rec0 + t.sk).x;
+        j_ket = b.tb > tk;
+        j_bra = b.tb < tk;
+    }
+    if (!t.skipJ) {
+        if (j_ket) {
+            for (int u = tl; u < NRS; u += S) {
+                double acc = 0.0;
+#pragma unroll 4
+                for (int pq = 0; pq < NPQ; ++pq) acc += (double)I[pq * NRS + u] * L.Ppq[pq];
+                unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
+            }
+        }
+        if (j_bra) {
+#pragma unroll
+            for (int m = 0; m < MJ; ++m) {
+                const int pq = tl + S * m;
+                if (pq < NPQ) {
+                    double acc = 0.0;
+#pragma unroll 4
+                    for (int rs = 0; rs < NRS; ++rs) acc += (double)I[pq * NRS + rs] * L.Prs[rs];
+                    Jpq[m] += f * acc;
+                }
+            }
+        }
+    }
+    if (!t.skipK) {
+        constexpr int N1 = NP * NR, N2 = N1 + NP * NS, N3 = N2 + NQ * NR, N4 = N3 + NQ * NS;
+        for (int u = tl; u < N4; u += S) {
+            double acc = 0.0;
+            size_t dst;
+            if (u < N1) {  // K(p,r) += I P(s,q)
+                const int p = u / NR, r = u % NR;
+#pragma unroll 1
+                for (int q = 0; q < NQ; ++q)
+#pragma unroll 4
+                    for (int s = 0; s < NS; ++s) acc += (double)I[(p * NQ + q) * NRS + r * NS + s] * L.Psq[s * NQ + q];
+                dst = (size_t)(fp + p) * ld + fr + r;
+            } else if (u < N2) {  // K(p,s) += I P(r,q)
+                const int p = (u - N1) / NS, s = (u - N1) % NS;
+#pragma unroll 1
+                for (int q = 0; q < NQ; ++q)
+#pragma unroll 4
+                    for (int r = 0; r < NR; ++r) acc += (double)I[(p * NQ + q) * NRS + r * NS + s] * L.Prq[r * NQ + q];
+                dst = (size_t)(fp + p) * ld + fs + s;
+            } else if (u < N3) {  // K(q,r) += I P(s,p)
+                const int q = (u - N2) / NR, r = (u - N2) % NR;
+#pragma unroll 1
+                for (int p = 0; p < NP; ++p)
+#pragma unroll 4
+                    for (int s = 0; s < NS; ++s) acc += (double)I[(p * NQ + q) * NRS + r * NS + s] * L.Psp[s * NP + p];
+                dst = (size_t)(fq + q) * ld + fr + r;
+            } else {  // K(q,s) += I P(r,p)
+                const int q = (u - N3) / NS, s = (u - N3) % NS;
+#pragma unroll 1
+                for (int p = 0; p < NP; ++p)
+#pragma unroll 4
+                    for (int r = 0; r < NR; ++r) acc += (double)I[(p * NQ + q) * NRS + r * NS + s] * L.Prp[r * NP + p];
+                dst = (size_t)(fq + q) * ld + fs + s;
+            }
+            unsafeAtomicAdd(&b.Y[dst], f * acc);
+        }
+    }
+}
+
+// The wavefront form's work for bra pair bi: the chunks blockIdx.y, blockIdx.y + gridDim.y,
+// ... of a.ket_chunk kets of its Schwarz prefix (only chunks that hold kets are visited).
+// The 64 lanes screen up to 64 kets of a chunk at once; the kets that pass are then taken
+// TEAMS at a time, one per team (their test results fetched from the lanes that computed
+// them).  The prefix is split where max_I(ab) max_I(cd) falls below a.fp32_thr (0 in an
+// FP64 build: no split): the FP64 instantiation (R = double) takes the kets before the
+// split, the FP32 one (R = float: tables, accumulation and transfers in FP32) those after
+// it, in a launch of its own, so that neither carries the other's registers.
+template <int LP, int LQ, int LR, int LS, typename R>
+__device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> *Lt, double *gbuf)
+{
+    using T = WaveTab<LP, LQ, LR, LS>;
+    using C = WaveCfg<LP, LQ, LR, LS>;
+    constexpr bool F32 = sizeof(R) == sizeof(float);
+    constexpr int NP = T::NP, NQ = T::NQ, NR = T::NR, NS = T::NS, S = C::S, NT = C::TEAMS;
+    constexpr int NPQ = NP * NQ, NRS = NR * NS, MJ = (NPQ + S - 1) / S;
+    const int lane = threadIdx.x, team = lane / S, tl = lane - team * S;
+    WaveLDS<LP, LQ, LR, LS> &L = Lt[team];
     BraInfo b;
     load_bra(a, bi, b);
+    int lo = 0, hi = b.jend;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (b.mIb * ldc(b.maxI + mid) < a.fp32_thr) hi = mid;
+        else lo = mid + 1;
+    }
+    const int jbeg = F32 ? lo : 0, jend = F32 ? b.jend : lo;
     const int step = gridDim.y * a.ket_chunk;
-    if ((int)blockIdx.y * a.ket_chunk >= b.jend) return;  // uniform over the work-group
+    if (jbeg + (int)blockIdx.y * a.ket_chunk >= jend) return;  // uniform over the work-group
     const int ld = b.ld, fp = b.fp, fq = b.fq;
     const double *__restrict__ P = b.P;
-    for (int t = lane; t < NPQ; t += 64) L.Ppq[t] = P[(size_t)(fp + t / NQ) * ld + fq + t % NQ];
+    for (int t = tl; t < NPQ; t += S) L.Ppq[t] = P[(size_t)(fp + t / NQ) * ld + fq + t % NQ];
     double Jpq[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; ++m) Jpq[m] = 0.0;
     unsigned long long nq = 0, npq = 0;
-    // the 64 lanes screen up to 64 kets of a chunk at once; the kets that pass are then
-    // taken one after the other by the whole wavefront (their test results broadcast from
-    // the lane that computed them)
-    for (int jc = blockIdx.y * a.ket_chunk; jc < b.jend; jc += step) {
-        const int jlim = min(b.jend, jc + a.ket_chunk);
-        KetTest tl;
-        const bool ok = jc + lane < jlim && test_ket(a, b, jc + lane, tl);
-        const int flags_l = ok ? ((int)tl.skipJ | ((int)tl.skipK << 1) | ((int)tl.skip_lists << 2) |
-                                  (tl.halvings << 3))
+    for (int jc = jbeg + blockIdx.y * a.ket_chunk; jc < jend; jc += step) {
+        const int jlim = min(jend, jc + a.ket_chunk);
+        KetTest tt;
+        const bool ok = jc + lane < jlim && test_ket(a, b, jc + lane, tt);
+        const int flags_l = ok ? ((int)tt.skipJ | ((int)tt.skipK << 1) | ((int)tt.skip_lists << 2) |
+                                  (tt.halvings << 3))
                                : 0;
-        const int frs_l = ok ? pack16(tl.fr, tl.fs) : 0;
+        const int frs_l = ok ? pack16(tt.fr, tt.fs) : 0;
         unsigned long long pass = __ballot(ok);
     while (pass) {  // uniform over the work-group
-        const int l = __builtin_ctzll(pass);
-        pass &= pass - 1;
-        const int j = jc + l;
-        const int flags = __builtin_amdgcn_readlane(flags_l, l), frs = __builtin_amdgcn_readlane(frs_l, l);
+        // this round: the team-th remaining ket for each team
+        const int l = nth_set_bit(pass, team);
+        const bool live = l < 64;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) pass &= pass - 1;
+        const int src_lane = live ? l : 0;
+        const int flags = __shfl(flags_l, src_lane, 64), frs = __shfl(frs_l, src_lane, 64);
+        const int j = jc + src_lane;
         KetTest t;
         t.sk = b.kb + j;
         t.skipJ = flags & 1;
@@ -938,111 +1139,66 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
         t.f = (flags >> 3) == 0 ? 1.0 : (flags >> 3) == 1 ? 0.5 : (flags >> 3) == 2 ? 0.25 : 0.125;
         const PairView bv = make_view(a.pr, a.pm, b.sb, t.skip_lists);
         const PairView kv = make_view(a.pr, a.pm, t.sk, t.skip_lists);
-        const double f = t.f;
         const int fr = hi16(frs), fs = lo16(frs);
         // stage the ket-dependent density blocks (P symmetric: the K-side blocks come from
-        // the bra pair's rows); when each fits one pass of the 64 lanes the five loads are
-        // issued together and waited for once
-        if constexpr (NRS <= 64 && NS * NQ <= 64 && NR * NQ <= 64 && NS * NP <= 64 && NR * NP <= 64) {
-            const int u = lane;
-            const double v0 = u < NRS ? P[(size_t)(fr + u / NS) * ld + fs + u % NS] : 0.0;
-            const double v1 = u < NS * NQ ? P[(size_t)(fq + u % NQ) * ld + fs + u / NQ] : 0.0;
-            const double v2 = u < NR * NQ ? P[(size_t)(fq + u % NQ) * ld + fr + u / NQ] : 0.0;
-            const double v3 = u < NS * NP ? P[(size_t)(fp + u % NP) * ld + fs + u / NP] : 0.0;
-            const double v4 = u < NR * NP ? P[(size_t)(fp + u % NP) * ld + fr + u / NP] : 0.0;
-            if (u < NRS) L.Prs[u] = v0;
-            if (u < NS * NQ) L.Psq[u] = v1;
-            if (u < NR * NQ) L.Prq[u] = v2;
-            if (u < NS * NP) L.Psp[u] = v3;
-            if (u < NR * NP) L.Prp[u] = v4;
-        } else {
-            for (int u = lane; u < NRS; u += 64) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
-            for (int u = lane; u < NS * NQ; u += 64) L.Psq[u] = P[(size_t)(fq + u % NQ) * ld + fs + u / NQ];
-            for (int u = lane; u < NR * NQ; u += 64) L.Prq[u] = P[(size_t)(fq + u % NQ) * ld + fr + u / NQ];
-            for (int u = lane; u < NS * NP; u += 64) L.Psp[u] = P[(size_t)(fp + u % NP) * ld + fs + u / NP];
-            for (int u = lane; u < NR * NP; u += 64) L.Prp[u] = P[(size_t)(fp + u % NP) * ld + fr + u / NP];
-        }
-        const double *I = wave_eri<LP, LQ, LR, LS>(a.pm, bv, kv, L, gbuf);
-        ++nq;
-        npq += (unsigned long long)(bv.n * kv.n);
-        // J sides as in the lane form (both, or the UHF pass's Tonto ket side only)
-        bool j_ket = true, j_bra = true;
-        if (a.j_ket_nonsame) {
-            const int tk = ldc(a.pr.rec0 + t.sk).x;
-            j_ket = b.tb > tk;
-            j_bra = b.tb < tk;
-        }
-        if (!t.skipJ) {
-            if (j_ket) {
-                for (int u = lane; u < NRS; u += 64) {
-                    double acc = 0.0;
-#pragma unroll 4
-                    for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + u] * L.Ppq[pq];
-                    unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
-                }
+        // the bra pair's rows); when each fits one pass of the team's lanes the five loads
+        // are issued together and waited for once
+        if (live) {
+            if constexpr (NRS <= S && NS * NQ <= S && NR * NQ <= S && NS * NP <= S && NR * NP <= S) {
+                const int u = tl;
+                const double v0 = u < NRS ? P[(size_t)(fr + u / NS) * ld + fs + u % NS] : 0.0;
+                const double v1 = u < NS * NQ ? P[(size_t)(fq + u % NQ) * ld + fs + u / NQ] : 0.0;
+                const double v2 = u < NR * NQ ? P[(size_t)(fq + u % NQ) * ld + fr + u / NQ] : 0.0;
+                const double v3 = u < NS * NP ? P[(size_t)(fp + u % NP) * ld + fs + u / NP] : 0.0;
+                const double v4 = u < NR * NP ? P[(size_t)(fp + u % NP) * ld + fr + u / NP] : 0.0;
+                if (u < NRS) L.Prs[u] = v0;
+                if (u < NS * NQ) L.Psq[u] = v1;
+                if (u < NR * NQ) L.Prq[u] = v2;
+                if (u < NS * NP) L.Psp[u] = v3;
+                if (u < NR * NP) L.Prp[u] = v4;
+            } else {
+                for (int u = tl; u < NRS; u += S) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
+                for (int u = tl; u < NS * NQ; u += S) L.Psq[u] = P[(size_t)(fq + u % NQ) * ld + fs + u / NQ];
+                for (int u = tl; u < NR * NQ; u += S) L.Prq[u] = P[(size_t)(fq + u % NQ) * ld + fr + u / NQ];
+                for (int u = tl; u < NS * NP; u += S) L.Psp[u] = P[(size_t)(fp + u % NP) * ld + fs + u / NP];
+                for (int u = tl; u < NR * NP; u += S) L.Prp[u] = P[(size_t)(fp + u % NP) * ld + fr + u / NP];
             }
-            if (j_bra) {
+        }
+        // primitive quartets of the longest quartet this round (uniform loop bound)
+        const int np = live ? bv.n * kv.n : 0;
+        int npmax = np;
 #pragma unroll
-                for (int m = 0; m < MJ; ++m) {
-                    const int pq = lane + 64 * m;
-                    if (pq < NPQ) {
-                        double acc = 0.0;
-#pragma unroll 4
-                        for (int rs = 0; rs < NRS; ++rs) acc += I[pq * NRS + rs] * L.Prs[rs];
-                        Jpq[m] += f * acc;
-                    }
-                }
-            }
-        }
-        if (!t.skipK) {
-            constexpr int N1 = NP * NR, N2 = N1 + NP * NS, N3 = N2 + NQ * NR, N4 = N3 + NQ * NS;
-            for (int u = lane; u < N4; u += 64) {
-                double acc = 0.0;
-                size_t dst;
-                if (u < N1) {  // K(p,r) += I P(s,q)
-                    const int p = u / NR, r = u % NR;
-#pragma unroll 1
-                    for (int q = 0; q < NQ; ++q)
-#pragma unroll 4
-                        for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psq[s * NQ + q];
-                    dst = (size_t)(fp + p) * ld + fr + r;
-                } else if (u < N2) {  // K(p,s) += I P(r,q)
-                    const int p = (u - N1) / NS, s = (u - N1) % NS;
-#pragma unroll 1
-                    for (int q = 0; q < NQ; ++q)
-#pragma unroll 4
-                        for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prq[r * NQ + q];
-                    dst = (size_t)(fp + p) * ld + fs + s;
-                } else if (u < N3) {  // K(q,r) += I P(s,p)
-                    const int q = (u - N2) / NR, r = (u - N2) % NR;
-#pragma unroll 1
-                    for (int p = 0; p < NP; ++p)
-#pragma unroll 4
-                        for (int s = 0; s < NS; ++s) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Psp[s * NP + p];
-                    dst = (size_t)(fq + q) * ld + fr + r;
-                } else {  // K(q,s) += I P(r,p)
-                    const int q = (u - N3) / NS, s = (u - N3) % NS;
-#pragma unroll 1
-                    for (int p = 0; p < NP; ++p)
-#pragma unroll 4
-                        for (int r = 0; r < NR; ++r) acc += I[(p * NQ + q) * NRS + r * NS + s] * L.Prp[r * NP + p];
-                    dst = (size_t)(fq + q) * ld + fs + s;
-                }
-                unsafeAtomicAdd(&b.Y[dst], f * acc);
+        for (int off = 32; off >= S; off >>= 1) npmax = max(npmax, __shfl_xor(npmax, off, 64));
+        const R *I = wave_eri<LP, LQ, LR, LS, S, R>(a.pm, bv, kv, wave_buf<LP, LQ, LR, LS, R>(L),
+                                                    reinterpret_cast<R *>(gbuf), tl, live, npmax);
+        if (live) {
+            wave_digest<LP, LQ, LR, LS, S, R>(a, b, t, fr, fs, I, L, tl, Jpq);
+            if (tl == 0) {
+                ++nq;
+                npq += (unsigned long long)np;
             }
         }
         __syncthreads();
     }
     }
+    // the teams' partial sums of the bra-pair J block: lanes with the same tl add up
 #pragma unroll
     for (int m = 0; m < MJ; ++m) {
-        const int pq = lane + 64 * m;
-        if (pq < NPQ && Jpq[m] != 0.0)
+#pragma unroll
+        for (int off = 32; off >= S; off >>= 1) Jpq[m] += __shfl_xor(Jpq[m], off, 64);
+        const int pq = tl + S * m;
+        if (team == 0 && pq < NPQ && Jpq[m] != 0.0)
             unsafeAtomicAdd(&b.X[(size_t)(fp + pq / NQ) * ld + fq + pq % NQ], Jpq[m]);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        nq += __shfl_xor(nq, off, 64);
+        npq += __shfl_xor(npq, off, 64);
     }
     if (lane == 0 && nq) {
         atomicAdd(&a.counters[3 * a.counter_slot + 0], nq);
         atomicAdd(&a.counters[3 * a.counter_slot + 1], npq);
+        if (F32) atomicAdd(&a.counters[3 * a.counter_slot + 2], nq);
     }
     __syncthreads();  // L is reused by the work-group's next bra pair
 }
@@ -1050,19 +1206,22 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> &L, do
 // Wavefront form: grid (gx, gy); work-group (x, y) takes bra pairs x, x + gx, ... and, of
 // each, the ket chunks y, y + gy, ...  Workspace classes keep gx * gy <= their slot count and use
 // slot y * gx + x.
-template <int LP, int LQ, int LR, int LS>
-__global__ __launch_bounds__(64) void jk_wave_kernel(JKArgs a)
+#ifndef TGPU_WAVE_MIN_WAVES
+#define TGPU_WAVE_MIN_WAVES 1
+#endif
+template <int LP, int LQ, int LR, int LS, typename R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TGPU_WAVE_MIN_WAVES, 8))) void jk_wave_kernel(JKArgs a)
 {
     using C = WaveCfg<LP, LQ, LR, LS>;
-    __shared__ WaveLDS<LP, LQ, LR, LS> L;
+    __shared__ WaveLDS<LP, LQ, LR, LS> L[C::TEAMS];
     double *gbuf = nullptr;
     if constexpr (C::GB) gbuf = a.ws + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * C::WS;
-    for (int bi = blockIdx.x; bi < a.n_bra; bi += gridDim.x) wave_bra<LP, LQ, LR, LS>(a, bi, L, gbuf);
+    for (int bi = blockIdx.x; bi < a.n_bra; bi += gridDim.x) wave_bra<LP, LQ, LR, LS, R>(a, bi, L, gbuf);
 }
 
 template <int LP, int LQ>
 __global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims pm, int start, int n,
-                                                         double *maxI, double *ws)
+                                                         double *maxI, double *ws, int sph)
 {
     using T = WaveTab<LP, LQ, LP, LQ>;
     using C = WaveCfg<LP, LQ, LP, LQ>;
@@ -1073,9 +1232,14 @@ __global__ __launch_bounds__(64) void schwarz_wave_kernel(DevPairs pr, DevPrims 
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int s = start + i;
         const PairView v = make_view(pr, pm, s, false);
-        const double *I = wave_eri<LP, LQ, LP, LQ>(pm, v, v, L, gbuf);
+        const double *I = wave_eri<LP, LQ, LP, LQ, 64, double>(pm, v, v, L.u.d, gbuf, threadIdx.x, true, v.n * v.n);
         double m = 0.0;
-        for (int t = threadIdx.x; t < NPQ; t += 64) m = fmax(m, fabs(I[t * NPQ + t]));
+        if (sph && (LP >= 2 || LQ >= 2)) {
+            for (int t = threadIdx.x; t < nsph(LP) * nsph(LQ); t += 64)
+                m = fmax(m, fabs(sph_abab<LP, LQ>(I, t / nsph(LQ), t % nsph(LQ))));
+        } else {
+            for (int t = threadIdx.x; t < NPQ; t += 64) m = fmax(m, fabs(I[t * NPQ + t]));
+        }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
         if (threadIdx.x == 0) maxI[s] = sqrt(m);
@@ -1092,6 +1256,6 @@ TGPU_FOR_EACH_CLASS(TGPU_DECLARE_JK)
 #undef TGPU_DECLARE_JK
 #define TGPU_DECLARE_SCHWARZ(LP, LQ, H) \
     void tgpu_launch_schwarz_##LP##LQ(const DevPairs &pr, const DevPrims &pm, int start, int n, double *maxI, \
-                                      double *ws, long long ws_doubles, hipStream_t s);
+                                      double *ws, long long ws_doubles, int sph, hipStream_t s);
 TGPU_FOR_EACH_PAIRCLASS(TGPU_DECLARE_SCHWARZ)
 #undef TGPU_DECLARE_SCHWARZ

@@ -52,6 +52,8 @@ struct BatchArgs {
     const long long *mat_off;    // [n_mol]
     const int *nbf;              // [n_mol]
     const int *sh_l, *sh_bf;     // global shell index -> l, first basis function (local)
+    const int *sh_sbf;           // spherical basis (one molecule): first spherical function, else null
+    int n_sph;                   // spherical functions (the spherical P's leading dimension)
 };
 
 __device__ __forceinline__ int mol_of(const long long *off, int n, long long g)
@@ -84,15 +86,19 @@ __global__ void pmax_kernel(const double *__restrict__ P, const double *__restri
         const int m = mol_of(b.pair_off, b.n_mol, g);
         int x, y;
         decode_pair(g - b.pair_off[m], x, y);
-        const int s0 = b.shell_off[m], ld = b.nbf[m];
-        const double *Pm = P + b.mat_off[m];
-        const int fa = b.sh_bf[s0 + x], na = ncart(b.sh_l[s0 + x]);
-        const int fb = b.sh_bf[s0 + y], nb = ncart(b.sh_l[s0 + y]);
+        const int s0 = b.shell_off[m];
+        const bool sph = b.sh_sbf != nullptr;
+        const int ld = sph ? b.n_sph : b.nbf[m];
+        const double *Pm = P + (sph ? 0 : b.mat_off[m]);
+        const int fa = sph ? b.sh_sbf[s0 + x] : b.sh_bf[s0 + x];
+        const int fb = sph ? b.sh_sbf[s0 + y] : b.sh_bf[s0 + y];
+        const int na = sph ? nsph(b.sh_l[s0 + x]) : ncart(b.sh_l[s0 + x]);
+        const int nb = sph ? nsph(b.sh_l[s0 + y]) : ncart(b.sh_l[s0 + y]);
         double v = 0.0;
         for (int i = 0; i < na; ++i)
             for (int j = 0; j < nb; ++j) v = fmax(v, fabs(Pm[(size_t)(fa + i) * ld + fb + j]));
         if (P2) {
-            const double *P2m = P2 + b.mat_off[m];
+            const double *P2m = P2 + (sph ? 0 : b.mat_off[m]);
             for (int i = 0; i < na; ++i)
                 for (int j = 0; j < nb; ++j) v = fmax(v, fabs(P2m[(size_t)(fa + i) * ld + fb + j]));
         }
@@ -151,6 +157,60 @@ __global__ void post_kernel(const double *__restrict__ X, const double *__restri
         const size_t i = idx / n, j = idx % n, t = j * n + i;
         J[mo + idx] = 2.0 * (X[mo + idx] + X[mo + t]);
         K[mo + idx] = Y[mo + idx] + Y[mo + t];
+    }
+}
+
+// Spherical basis (one molecule): Cartesian <-> real solid harmonic transforms of a
+// symmetric matrix.  T is block diagonal (one (ncart x nsph) block per shell, the unit
+// matrix for s and p): P_cart = T P_sph T^T before a build, J_sph = T^T J_cart T after
+// it, so the Cartesian kernels with P_cart give exactly the spherical J and K of the
+// per-quartet transform (SHELL1QUARTET:change_to_spherical, shell1quartet.foo:505-563).
+// One thread per output element; bf_* map a function to its shell and component.
+struct SphMaps {
+    const int *cbf_shell, *sbf_shell;  // Cartesian / spherical function -> shell
+    const int *sh_l, *sh_cbf, *sh_sbf;  // shell -> l, first Cartesian, first spherical
+    int n_cart, n_sph;
+};
+
+__global__ void sph_to_cart_kernel(const double *__restrict__ Ps, double *__restrict__ Pc, SphMaps m)
+{
+    const size_t nn = (size_t)m.n_cart * m.n_cart;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx / m.n_cart), j = (int)(idx - (size_t)i * m.n_cart);
+        const int si = m.cbf_shell[i], sj = m.cbf_shell[j];
+        const int li = m.sh_l[si], lj = m.sh_l[sj];
+        const int ci = i - m.sh_cbf[si], cj = j - m.sh_cbf[sj];
+        const double *row = Ps + (size_t)m.sh_sbf[si] * m.n_sph + m.sh_sbf[sj];
+        double v = 0.0;
+        for (int a = 0; a < nsph(li); ++a) {
+            const double ta = sph_t(li, ci, a);
+            if (ta == 0.0) continue;
+            double w = 0.0;
+            for (int b = 0; b < nsph(lj); ++b) w += sph_t(lj, cj, b) * row[(size_t)a * m.n_sph + b];
+            v += ta * w;
+        }
+        Pc[idx] = v;
+    }
+}
+
+__global__ void cart_to_sph_kernel(const double *__restrict__ Jc, double *__restrict__ Js, SphMaps m)
+{
+    const size_t nn = (size_t)m.n_sph * m.n_sph;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < nn; idx += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx / m.n_sph), j = (int)(idx - (size_t)i * m.n_sph);
+        const int si = m.sbf_shell[i], sj = m.sbf_shell[j];
+        const int li = m.sh_l[si], lj = m.sh_l[sj];
+        const int ai = i - m.sh_sbf[si], aj = j - m.sh_sbf[sj];
+        const double *row = Jc + (size_t)m.sh_cbf[si] * m.n_cart + m.sh_cbf[sj];
+        double v = 0.0;
+        for (int c = 0; c < ncart(li); ++c) {
+            const double tc = sph_t(li, c, ai);
+            if (tc == 0.0) continue;
+            double w = 0.0;
+            for (int d = 0; d < ncart(lj); ++d) w += sph_t(lj, d, aj) * row[(size_t)c * m.n_cart + d];
+            v += tc * w;
+        }
+        Js[idx] = v;
     }
 }
 
@@ -294,6 +354,23 @@ static const bool kHeavy[NQCLS] = {
 #undef TGPU_HEAVY_FLAG
 };
 
+// Light class pairs whose wavefront form is the default: their lane-per-quartet kernels
+// need 230-380 registers (one or two wavefronts per SIMD), and the wavefront form was the
+// faster one on (H2O)64 / cc-pVTZ (tgpu_autotune, FP64 build at an SCF density: (pp|pp)
+// 127 vs 171 ms, (ds|pp) 424 vs 570, (ds|ds) 379 vs 475, (fs|ps) 402 vs 458, (fs|ds) 233
+// vs 359).  tgpu_autotune re-decides per molecule.
+static bool wave_by_default(int lp, int lq, int lr, int ls)
+{
+    return (lp == 1 && lq == 1 && lr == 1 && ls == 1) || (lp == 2 && lq == 0 && lr == 1 && ls == 1) ||
+           (lp == 2 && lq == 0 && lr == 2 && ls == 0) || (lp == 3 && lq == 0 && lr == 1 && ls == 0) ||
+           (lp == 3 && lq == 0 && lr == 2 && ls == 0);
+}
+static const bool kWaveDefault[NQCLS] = {
+#define TGPU_WAVE_DEFAULT(LP, LQ, LR, LS, H) (H || wave_by_default(LP, LQ, LR, LS)),
+    TGPU_FOR_EACH_CLASS(TGPU_WAVE_DEFAULT)
+#undef TGPU_WAVE_DEFAULT
+};
+
 struct tgpu_ctx {
     std::string err;
     int device = 0;
@@ -340,6 +417,13 @@ struct tgpu_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> cls_ev;  // 2 per class pair
     std::vector<char> cls_launched;
+    // spherical basis (Tonto's use_spherical_basis; one molecule): P, J, K, S, T, V are
+    // n_sph x n_sph and the build is make_r_JK_direct's
+    bool spherical = false;
+    int n_sph = 0;
+    std::vector<int> sh_sbf, cbf_shell, sbf_shell;
+    DBuf<int> d_sh_sbf, d_cbf_shell, d_sbf_shell;
+    DBuf<double> d_Pc, d_Jc, d_Kc;  // the Cartesian P, J, K of a spherical build
     // captured build
     hipGraphExec_t gexec = nullptr;
     const void *g_key[3] = {nullptr, nullptr, nullptr};
@@ -523,7 +607,24 @@ static BatchArgs batch_args(const tgpu_ctx *c)
     b.n_mol = c->n_mol; b.pair_off = c->d_pair_off.p; b.shell_off = c->d_shell_off.p;
     b.atom_off = c->d_atom_off.p; b.mat_off = c->d_mat_off.p; b.nbf = c->d_nbf.p;
     b.sh_l = c->d_sh_l.p; b.sh_bf = c->d_sh_bf.p;
+    b.sh_sbf = nullptr;
+    b.n_sph = 0;
     return b;
+}
+
+static SphMaps sph_maps(const tgpu_ctx *c)
+{
+    SphMaps m;
+    m.cbf_shell = c->d_cbf_shell.p; m.sbf_shell = c->d_sbf_shell.p;
+    m.sh_l = c->d_sh_l.p; m.sh_cbf = c->d_sh_bf.p; m.sh_sbf = c->d_sh_sbf.p;
+    m.n_cart = c->nbf[0]; m.n_sph = c->n_sph;
+    return m;
+}
+
+// elements of one matrix of the caller (all molecules of a batch; spherical: n_sph^2)
+static size_t caller_elems(const tgpu_ctx *c)
+{
+    return c->spherical ? (size_t)c->n_sph * c->n_sph : (size_t)c->n_elem;
 }
 
 // Permute all per-pair host arrays by `perm` (new position -> old position).
@@ -685,7 +786,7 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
         }
     }
     build_prims(c);
-    for (int k = 0; k < NQCLS; ++k) c->variant[k] = kHeavy[k] ? 1 : 0;
+    for (int k = 0; k < NQCLS; ++k) c->variant[k] = kWaveDefault[k] ? 1 : 0;
     int rc;
     if ((rc = upload_pairs(c))) return rc;
     if ((rc = dupload(c, c->d_sh_l, c->sh_l)) || (rc = dupload(c, c->d_sh_bf, c->sh_bf)) ||
@@ -723,6 +824,8 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_hP2); dfree(c->d_hJ2); dfree(c->d_hK2); dfree(c->d_uJ); dfree(c->d_uK);
     dfree(c->d_counters);
     dfree(c->d_ws);
+    dfree(c->d_sh_sbf); dfree(c->d_cbf_shell); dfree(c->d_sbf_shell);
+    dfree(c->d_Pc); dfree(c->d_Jc); dfree(c->d_Kc);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
@@ -741,7 +844,7 @@ extern "C" const char *tgpu_last_error(const tgpu_ctx *c)
 extern "C" int tgpu_dims(const tgpu_ctx *c, int *n_bf, long long *n_pairs)
 {
     if (!c) return TGPU_E_ARG;
-    if (n_bf) *n_bf = c->nbf[0];
+    if (n_bf) *n_bf = c->spherical ? c->n_sph : c->nbf[0];
     if (n_pairs) *n_pairs = c->pair_off[1];
     return TGPU_OK;
 }
@@ -796,6 +899,41 @@ extern "C" int tgpu_set_fp32_threshold(tgpu_ctx *c, double thr)
     return TGPU_OK;
 }
 
+// Spherical basis (Tonto's use_spherical_basis): one molecule; the Schwarz table is redone
+// from the solid-harmonic (ab|ab) blocks on the next tgpu_schwarz or build.
+extern "C" int tgpu_set_spherical(tgpu_ctx *c, int on)
+{
+    if (!c) return TGPU_E_ARG;
+    if (on && c->n_mol != 1) {
+        c->err = "the spherical basis is supported for single-molecule contexts";
+        return TGPU_E_UNSUPPORTED;
+    }
+    if ((on != 0) == c->spherical) return TGPU_OK;
+    c->spherical = on != 0;
+    c->have_schwarz = false;
+    drop_graph(c);
+    if (!c->spherical) return TGPU_OK;
+    const int ns = (int)c->sh_l.size();
+    c->sh_sbf.assign(ns, 0);
+    c->cbf_shell.clear();
+    c->sbf_shell.clear();
+    int k = 0;
+    for (int s = 0; s < ns; ++s) {
+        c->sh_sbf[s] = k;
+        for (int i = 0; i < ncart(c->sh_l[s]); ++i) c->cbf_shell.push_back(s);
+        for (int i = 0; i < nsph(c->sh_l[s]); ++i) c->sbf_shell.push_back(s);
+        k += nsph(c->sh_l[s]);
+    }
+    c->n_sph = k;
+    const size_t nc = (size_t)c->nbf[0] * c->nbf[0];
+    int rc;
+    if ((rc = dupload(c, c->d_sh_sbf, c->sh_sbf)) || (rc = dupload(c, c->d_cbf_shell, c->cbf_shell)) ||
+        (rc = dupload(c, c->d_sbf_shell, c->sbf_shell)) || (rc = dalloc(c, c->d_Pc, nc)) ||
+        (rc = dalloc(c, c->d_Jc, nc)) || (rc = dalloc(c, c->d_Kc, nc)))
+        return rc;
+    return TGPU_OK;
+}
+
 typedef size_t (*ws_query)();
 static const ws_query kWaveWs[NQCLS] = {
 #define TGPU_WS_FN(LP, LQ, LR, LS, H) tgpu_wave_ws_##LP##LQ##LR##LS,
@@ -833,7 +971,7 @@ extern "C" int tgpu_schwarz(tgpu_ctx *c, double *max_I)
     {                                                                                    \
         const int b0 = pair_class(LP, LQ) * NBK, n = c->bin_start[b0 + NBK] - c->bin_start[b0]; \
         if (n) tgpu_launch_schwarz_##LP##LQ(dpr, dpm, c->bin_start[b0], n, c->d_maxI.p, c->d_ws.p, \
-                                            c->ws_doubles, c->stream);                        \
+                                            c->ws_doubles, (int)c->spherical, c->stream);    \
     }
         TGPU_FOR_EACH_PAIRCLASS(TGPU_SCHWARZ_CASE)
 #undef TGPU_SCHWARZ_CASE
@@ -917,11 +1055,24 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
                          int j_ket_nonsame = 0)
 {
     const bool mixed = flags & TGPU_JK_MIXED;
-    const BatchArgs ba = batch_args(c);
+    BatchArgs ba = batch_args(c);
+    // spherical basis: the caller's P, J, K are spherical; the kernels digest the
+    // Cartesian density T P T^T, the density maxima come from the spherical blocks
+    const double *Psph = P;
+    double *Jsph = J, *Ksph = K;
+    if (c->spherical) {
+        const SphMaps sm = sph_maps(c);
+        sph_to_cart_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(P, c->d_Pc.p, sm);
+        P = c->d_Pc.p;
+        J = c->d_Jc.p;
+        K = c->d_Kc.p;
+        ba.sh_sbf = c->d_sh_sbf.p;
+        ba.n_sph = c->n_sph;
+    }
     fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.X, c->n_elem);
     fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.Y, c->n_elem);
     fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(B.counters, 3 * NQCLS);
-    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(P, P2, ba, B.Pmax);
+    pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(c->spherical ? Psph : P, P2, ba, B.Pmax);
     mol_max_kernel<<<c->n_mol, 256, 0, s>>>(B.Pmax, ba, c->d_maxI_mol.p, B.scal);
     if (rec_start) (void)hipEventRecord(c->ev0, s);
     JKArgs a;
@@ -940,6 +1091,7 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     if (flags & TGPU_K_ONLY) { a.kp = 1e-20; a.jp = HUGE_VAL; }
     a.j_ket_nonsame = j_ket_nonsame;
     if (j_ket_nonsame) a.kp = HUGE_VAL;  // the pass digests no K
+    a.sph_screen = c->spherical ? 1 : 0;
     a.fp32_thr = c->fp32_thr;
     a.X = B.X;
     a.Y = B.Y;
@@ -991,6 +1143,12 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     }
     if (rec_end) (void)hipEventRecord(c->ev1, s);
     post_kernel<<<dim3(grid_for(c->n_elem / c->n_mol + 1), c->n_mol), 256, 0, s>>>(B.X, B.Y, ba, J, K);
+    if (c->spherical) {
+        const SphMaps sm = sph_maps(c);
+        const size_t ns2 = (size_t)c->n_sph * c->n_sph;
+        cart_to_sph_kernel<<<grid_for(ns2), 256, 0, s>>>(J, Jsph, sm);
+        cart_to_sph_kernel<<<grid_for(ns2), 256, 0, s>>>(K, Ksph, sm);
+    }
     return TGPU_OK;
 }
 
@@ -1004,6 +1162,10 @@ extern "C" int tgpu_make_r_JK_device(tgpu_ctx *c, const double *P, double *J, do
     if ((flags & TGPU_J_ONLY) && (flags & TGPU_K_ONLY)) {
         c->err = "TGPU_J_ONLY and TGPU_K_ONLY are exclusive";
         return TGPU_E_ARG;
+    }
+    if (c->spherical && (flags & (TGPU_J_ONLY | TGPU_K_ONLY))) {
+        c->err = "J-only / K-only builds are Cartesian (make_r_J_engine); the spherical build is make_r_JK_direct";
+        return TGPU_E_UNSUPPORTED;
     }
     const int kflags = flags & (TGPU_JK_MIXED | TGPU_J_ONLY | TGPU_K_ONLY);
     if (!(flags & TGPU_JK_GRAPH)) {
@@ -1067,6 +1229,10 @@ extern "C" int tgpu_make_r_JK_device_noalloc(tgpu_ctx *c, const double *P, doubl
     if ((flags & TGPU_J_ONLY) && (flags & TGPU_K_ONLY)) {
         c->err = "TGPU_J_ONLY and TGPU_K_ONLY are exclusive";
         return TGPU_E_ARG;
+    }
+    if (c->spherical) {
+        c->err = "tgpu_make_r_JK_device_noalloc: Cartesian bases only";
+        return TGPU_E_UNSUPPORTED;
     }
     auto up = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
     const size_t b_mat = up((size_t)c->n_elem * sizeof(double));
@@ -1137,6 +1303,10 @@ extern "C" int tgpu_make_u_JK_device(tgpu_ctx *c, const double *Pa, const double
         c->err = "tgpu_make_u_JK: only TGPU_JK_MIXED is supported";
         return TGPU_E_ARG;
     }
+    if (c->spherical) {
+        c->err = "tgpu_make_u_JK: the spherical basis is supported by the RHF build only";
+        return TGPU_E_UNSUPPORTED;
+    }
     if (!c->have_schwarz) {
         int rc = tgpu_schwarz(c, nullptr);
         if (rc) return rc;
@@ -1181,7 +1351,7 @@ extern "C" int tgpu_make_u_JK(tgpu_ctx *c, const double *Pa, const double *Pb, d
 extern "C" int tgpu_make_r_JK(tgpu_ctx *c, const double *P, double *J, double *K, int flags)
 {
     if (!c || !P || !J || !K) return TGPU_E_ARG;
-    const size_t nn = (size_t)c->n_elem;
+    const size_t nn = caller_elems(c);
     int rc;
     if ((rc = dalloc(c, c->d_hP, nn)) || (rc = dalloc(c, c->d_hJ, nn)) || (rc = dalloc(c, c->d_hK, nn)))
         return rc;
@@ -1345,9 +1515,20 @@ extern "C" int tgpu_one_electron(tgpu_ctx *c, double *S, double *T, double *V)
     a.xyz = d_xyz.p; a.Z = d_Z.p; a.S = d_S.p; a.T = d_T.p; a.V = d_V.p;
     one_e_kernel<<<(int)((c->n_pairs + 63) / 64), 64, 0, c->stream>>>(a);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(S, d_S.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(T, d_T.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(V, d_V.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    size_t nout = nn;
+    if (c->spherical && e == hipSuccess) {
+        // S, T, V of the solid harmonics: T^T M T, in place of the Cartesian blocks
+        const SphMaps sm = sph_maps(c);
+        nout = (size_t)c->n_sph * c->n_sph;
+        for (DBuf<double> *M : {&d_S, &d_T, &d_V}) {
+            cart_to_sph_kernel<<<grid_for(nout), 256, 0, c->stream>>>(M->p, c->d_Jc.p, sm);
+            e = hipMemcpyAsync(M->p, c->d_Jc.p, nout * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) break;
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(S, d_S.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(T, d_T.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(V, d_V.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     cleanup();
     if (e != hipSuccess) {

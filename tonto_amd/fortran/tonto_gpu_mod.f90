@@ -60,7 +60,39 @@ module tonto_gpu_mod
       end function
    end interface
 
+   interface
+      integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: s
+      end function
+   end interface
+
 contains
+
+   ! The text of tgpu_last_error(ctx) as a Fortran string.
+   function tgpu_error_message(ctx) result(msg)
+      type(c_ptr), intent(in) :: ctx
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer :: n, i
+      p = tgpu_last_error(ctx)
+      n = int(c_strlen(p))
+      call c_f_pointer(p, chars, [n])
+      allocate(character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = chars(i)
+      end do
+   end function
+
+   ! Tonto's DIE for a failed call: stop with the library's message.
+   subroutine tgpu_die(ctx, rc, what)
+      type(c_ptr), intent(in) :: ctx
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: what
+      write(*, '(a, a, i0, a, a)') what, ": libtonto_gpu error ", rc, ": ", tgpu_error_message(ctx)
+      error stop 1
+   end subroutine
 
    ! Replacement body for make_r_JK_engine(J,K,P): J and K come back after Tonto's
    ! post-processing, so make_r_Fock_mx's F = J - K/2 (+ core) is unchanged.
@@ -70,7 +102,7 @@ contains
       real(c_double), intent(out) :: J(:,:), K(:,:)
       integer(c_int) :: rc
       rc = tgpu_make_r_JK(ctx, P, J, K, 0_c_int)
-      if (rc /= TGPU_OK) stop "libtonto_gpu: make_r_JK failed (see tgpu_last_error)"
+      if (rc /= TGPU_OK) call tgpu_die(ctx, rc, "make_r_JK_engine")
    end subroutine
 
    ! Replacement body for make_u_JK_engine(J,K,P) (molecule.fock.foo:2558-2712):
@@ -81,7 +113,7 @@ contains
       real(c_double), intent(out) :: Ja(:,:), Jb(:,:), Ka(:,:), Kb(:,:)
       integer(c_int) :: rc
       rc = tgpu_make_u_JK(ctx, Pa, Pb, Ja, Jb, Ka, Kb, 0_c_int)
-      if (rc /= TGPU_OK) stop "libtonto_gpu: make_u_JK failed (see tgpu_last_error)"
+      if (rc /= TGPU_OK) call tgpu_die(ctx, rc, "make_u_JK_engine")
    end subroutine
 
 end module

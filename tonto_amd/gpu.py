@@ -19,7 +19,7 @@ TGPU_K_ONLY = 8
 
 # exported symbols, in the order of include/tonto_gpu.h (plus the two extensions)
 EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_set_stream",
-           "tgpu_set_cutoffs", "tgpu_set_fp32_threshold", "tgpu_schwarz", "tgpu_make_r_JK",
+           "tgpu_set_cutoffs", "tgpu_set_fp32_threshold", "tgpu_set_spherical", "tgpu_schwarz", "tgpu_make_r_JK",
            "tgpu_make_r_JK_device", "tgpu_make_r_JK_device_noalloc", "tgpu_make_u_JK", "tgpu_make_u_JK_device", "tgpu_get_stats",
            "tgpu_one_electron",
            "tgpu_get_class_counters", "tgpu_n_class_pairs", "tgpu_set_profiling",
@@ -65,6 +65,7 @@ def lib():
         L.tgpu_set_stream.argtypes = [vp, vp]
         L.tgpu_set_cutoffs.argtypes = [vp] + [ctypes.c_double] * 4
         L.tgpu_set_fp32_threshold.argtypes = [vp, ctypes.c_double]
+        L.tgpu_set_spherical.argtypes = [vp, ctypes.c_int]
         L.tgpu_schwarz.argtypes = [vp, vp]
         L.tgpu_make_r_JK.argtypes = [vp, dp, dp, dp, ctypes.c_int]
         L.tgpu_make_r_JK_device.argtypes = [vp, vp, vp, vp, ctypes.c_int]
@@ -101,7 +102,7 @@ class JKEngine:
     make_r_JK_engine).  For a batch, matrices are packed molecule after molecule."""
 
     def __init__(self, mol, schwarz=1e-9, j_density=1e-9, k_density=1e-9, pair=1e-6,
-                 fp32_threshold=1e-6):
+                 fp32_threshold=1e-6, spherical=False):
         mols = list(mol) if isinstance(mol, (list, tuple)) else [mol]
         self.mol = mol
         self.mols = mols
@@ -129,6 +130,14 @@ class JKEngine:
         self.cutoffs = (schwarz, j_density, k_density, pair)
         self._check(lib().tgpu_set_cutoffs(self._ctx, schwarz, j_density, k_density, pair))
         self._check(lib().tgpu_set_fp32_threshold(self._ctx, fp32_threshold))
+        self.spherical = bool(spherical)
+        if self.spherical:
+            # matrices in the solid-harmonic basis (Tonto's use_spherical_basis)
+            self._check(lib().tgpu_set_spherical(self._ctx, 1))
+            nb = ctypes.c_int()
+            self._check(lib().tgpu_dims(self._ctx, ctypes.byref(nb), None))
+            self.n_bf = nb.value
+            self.n_elements = self.n_bf * self.n_bf
 
     def _check(self, rc):
         if rc != 0:
@@ -193,6 +202,13 @@ class JKEngine:
         self._check(lib().tgpu_make_u_JK(self._ctx, _dp(Pa), _dp(Pb), *[_dp(x) for x in out],
                                          TGPU_JK_MIXED if mixed else 0))
         return tuple(x.reshape(n, n) for x in out)
+
+    def make_u_JK_device(self, Pa_ptr, Pb_ptr, Ja_ptr, Jb_ptr, Ka_ptr, Kb_ptr, mixed=False):
+        """UHF on device pointers, asynchronous on the stream (one molecule)."""
+        assert self.n_mol == 1
+        self._check(lib().tgpu_make_u_JK_device(self._ctx, *[ctypes.c_void_p(p) for p in
+                                                             (Pa_ptr, Pb_ptr, Ja_ptr, Jb_ptr, Ka_ptr, Kb_ptr)],
+                                                TGPU_JK_MIXED if mixed else 0))
 
     def make_r_JK_device(self, P_ptr, J_ptr, K_ptr, mixed=False, graph=False):
         """Device pointers (e.g. torch tensor .data_ptr()), asynchronous on the stream."""
