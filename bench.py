@@ -8,9 +8,11 @@ run in FP32 in the mixed builds.
 
 1. Warm-up (not timed): promolecule guess (atomic SCFs) and one loose build of each
    precision, so both SCF runs start from loaded kernels and cached atomic densities.
-2. FP64 SCF (full FP64 build every iteration) and mixed SCF (loose first builds, FP32
-   small-bound quartets, delta builds, FP64 from DIIS error 1e-4 on): wall time,
-   iterations, energy difference.  The FP64 build time comes from the FP64 SCF.
+2. FP64 SCF from the promolecule guess (full FP64 build every iteration) and the full
+   form: mixed SCF (loose first builds, FP32 small-bound quartets, delta builds, FP64 from
+   DIIS error 1e-4 on) from the learned guess (the 6-31G(d) network's density projected
+   onto cc-pVTZ): wall times with making the guess included, iterations, energy
+   difference.  The FP64 build time comes from the FP64 SCF.
 3. Timed steps: one step = one mixed-precision J/K build at the converged density with P,
    J and K resident in HBM; W warm-up steps, then K timed steps between device
    synchronisations.  value = contracted shell quartets computed per second.
@@ -259,17 +261,20 @@ def main():
                     max_iterations=a.profile_iterations, log=lambda s: log("profile " + s))
         P = r.P.contiguous()
     elif a.density is None:
-        # warm-up for both runs alike: atomic densities cached, every kernel loaded
+        # warm-up for both runs alike: atomic densities cached, every kernel loaded (the
+        # learned guess's network and projection included)
         t0 = time.perf_counter()
         P0 = scf.guess()
         scf.jk_scaled(P0, "full", 1e3, 1.0)
         scf.jk_scaled(P0, "full_mixed", 1e3, 1.0)
+        learned = scf.mlp_guess() is not None
         torch.cuda.synchronize()
         log(f"warm-up {time.perf_counter() - t0:.1f}s")
         r64 = scf.run(precision="fp64", convergence=1e-9, diis_tol=1e-6,
                       log=lambda s: log("fp64 " + s))
+        # the full form: mixed precision, incremental builds and the learned guess
         rmx = scf.run(precision="mixed", convergence=1e-9, diis_tol=1e-6,
-                      log=lambda s: log("mixed " + s))
+                      guess="mlp" if learned else "promolecule", log=lambda s: log("mixed " + s))
         dE = rmx.energy - r64.energy
         n64 = r64.n_builds["full"]
         extra.update({
@@ -283,6 +288,9 @@ def main():
             "fp64_build_s_mean_in_scf": r64.build_seconds["full"] / max(1, n64),
             "fp64_build_s_last_in_scf": r64.history[-1]["build_seconds"],
             "scf_converged": bool(r64.converged and rmx.converged),
+            "guess_fp64": "promolecule (atomic SCF densities)",
+            "guess_mixed": ("learned (6-31G(d) network, projected onto this basis)" if learned
+                            else "promolecule (no learned model)"),
             "mixed_schedule": {"loose_cutoff": scf.loose_cutoff, "switch_diis_error": scf.switch_error,
                                "mixed_delta_cutoffs": [scf.mixed_delta_cutoff, scf.mixed_delta_density_cutoff],
                                "fp64_delta_cutoffs": [scf.delta_cutoff, scf.delta_density_cutoff]}})

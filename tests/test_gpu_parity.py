@@ -58,12 +58,16 @@ def test_water_all_classes(torch_cuda, oracle_mod, basis):
     _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 1))
 
 
-@pytest.mark.parametrize("budget", [None, 20480])
+@pytest.mark.parametrize("budget", [None, 16690, 26450])
 def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
-    """The lane-per-quartet kernels' LDS K-row form on (H2O)_8 / 6-31G(d) (152 functions),
-    made to run for every ket segment length; with a 20 KiB LDS cap the (pp|ss), (pp|ps),
-    (ds|ss), (ds|ps) launches keep only 4 of their 6-7 K rows in LDS and update the rest
-    with global atomics."""
+    """The lane-per-quartet kernels' LDS K-row form on (H2O)_8 / 6-31G(d) (152 functions;
+    56 s-type, 48 p-type, 48 d-type), made to run for every ket segment length.  A row
+    holds the columns of the ket shells' momenta only (56 doubles for an (..|ss) class).
+    LDS caps: 16690 B leaves room next to the 14848 B two-root Rys table and ket queue for
+    4 rows of 56: (pp|ss) keeps 4 of 6 rows, (ds|ss) 4 of 7, (ps|ps) (104 columns) 2 of 4 in
+    LDS, the rest go to global atomics; 26450 B, next to the 22240 B of the three-root
+    classes: (dp|ps) (104 columns) keeps 5 of 9 rows, (dd|ss) 9 of 12, (fp|ss) 9 of 13,
+    (fd|ss) 9 of 16."""
     import os
     from tonto_amd.geometry import water_cluster
     g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))
@@ -80,6 +84,46 @@ def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_lds_row_form_every_lane_class(torch_cuda, oracle_mod, mixed):
+    """Every class pair that has the lane-per-quartet kernel, run in its LDS K-row form one
+    at a time (the others in the wavefront form), on (H2O)_2 / cc-pVTZ (130 functions, all
+    rows in LDS): J and K against the oracle to 1e-10 (FP64) and to the mixed-precision
+    bound 1e-8, the same quartets."""
+    from tonto_amd.build import class_list
+    from tonto_amd.flops import class_pairs
+    from tonto_amd.geometry import water_cluster
+    mol = water_cluster(2, "cc-pVTZ")
+    P = _rand_density(mol.n_bf, 31)
+    old = os.environ.get("TGPU_KROW_MIN_KETS")
+    os.environ["TGPU_KROW_MIN_KETS"] = "0"
+    try:
+        e = _engine(mol, schwarz=1e-11)
+    finally:
+        if old is None:
+            os.environ.pop("TGPU_KROW_MIN_KETS", None)
+        else:
+            os.environ["TGPU_KROW_MIN_KETS"] = old
+    mi = e.schwarz()
+    o = oracle_mod.OracleSystem(mol)
+    Jo, Ko, so = o.make_r_JK(P, schwarz=1e-11, max_I=mi, return_stats=True)
+    heavy = {tuple(c[:4]): c[4] for c in class_list()}
+    cps = class_pairs()
+    light = [k for k, cp in enumerate(cps) if not heavy[tuple(cp)]]
+    assert len(light) == 18
+    tol = 1e-8 if mixed else 1e-10
+    for k in light:
+        for kk in light:
+            e.set_variant(kk, 1)
+        e.set_variant(k, 0)
+        J, K = e.make_r_JK(P, mixed=mixed)
+        st = e.stats()
+        dJ, dK = np.max(np.abs(J - Jo)), np.max(np.abs(K - Ko))
+        print(f"class {cps[k]} mixed={mixed}: dJ={dJ:.2e} dK={dK:.2e}")
+        assert st["quartets"] == so["quartets"]
+        assert dJ <= tol and dK <= tol, cps[k]
 
 
 def test_f_shells_water_cc_pvtz(torch_cuda, oracle_mod):

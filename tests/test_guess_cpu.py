@@ -85,6 +85,26 @@ def test_frames_and_features_move_with_the_cluster():
     assert all(x.shape == (G.N_IN,) for x in [c1.features(0, 1, c1.frame(0, 1))])
 
 
+def test_whole_cluster_frames_and_features_equal_the_per_block_ones():
+    """Cluster.frames / features_batch (all blocks at once, environment sums per atom turned
+    into each frame) against frame() / features() block by block, on (H2O)_6 with its
+    pair blocks inside the 6 A cutoff."""
+    syms, xyz = _cluster(3, n=6)
+    cl = G.Cluster(syms, xyz)
+    for t, (a, b) in cl.block_index().items():
+        Q = cl.frames(a, b)
+        X = cl.features_batch(a, b, Q)
+        for i in range(len(a)):
+            F = cl.frame(int(a[i]), int(b[i]))
+            assert np.allclose(Q[i], F, atol=1e-14), (t, a[i], b[i])
+            assert np.allclose(X[i], cl.features(int(a[i]), int(b[i]), F), atol=1e-12), (t, a[i], b[i])
+    Q = cl.frames(np.array([0, 0]), np.array([0, 1]))
+    for l in (1, 2, 3):
+        Tb = G.cart_transform_batch(l, Q)
+        for i in range(2):
+            assert np.allclose(Tb[i], G.cart_transform(l, Q[i]), atol=1e-14)
+
+
 def test_assembled_guess_is_covariant_and_symmetric():
     """Any network (here random weights through the numpy checker) gives a guess that is
     symmetric and turns exactly with the cluster."""

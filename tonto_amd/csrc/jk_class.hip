@@ -23,13 +23,11 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
 #if !TGPU_HEAVY
     if (variant == 0) {
         // The LDS-row form for single molecules with long ket segments (>= 4 rounds of 64 by
-        // default, so that the four wavefronts of one bra pair have work), in the classes whose
-        // lane-per-quartet kernel stays well below the register limit: as many of the bra
-        // pair's K rows as fit the work-group's LDS budget next to the Rys table and the
-        // ket queue, and at least half of them.
+        // default, so that the four wavefronts of one bra pair have work): as many of the
+        // bra pair's K rows as fit the work-group's LDS budget next to the Rys table and
+        // the ket queue, and at least half of them.  A row holds the columns of the ket
+        // shells' two angular momenta only.
         constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
-#if (TGPU_LP + TGPU_LQ <= 1) || (TGPU_LP + TGPU_LQ == 2 && TGPU_LR + TGPU_LS <= 1) || \
-    (TGPU_LP + TGPU_LQ == 3 && TGPU_LR + TGPU_LS == 0)
         constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
         constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 128 * 8 : 0);
         constexpr size_t kMaxLds = 155 * 1024;
@@ -43,10 +41,13 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
             attr_set = true;
         }
         const size_t budget = a.krow_budget > 0 && (size_t)a.krow_budget < kMaxLds ? (size_t)a.krow_budget : kMaxLds;
-        const size_t row_bytes = (size_t)a.max_nbf * sizeof(double);
-        const int fit = a.max_nbf > 0 && budget > kStatic ? (int)((budget - kStatic) / row_bytes) : 0;
+        const int nR = a.kcol_lofs[TGPU_LR + 1] - a.kcol_lofs[TGPU_LR];
+        const int W = TGPU_LR == TGPU_LS ? nR : nR + a.kcol_lofs[TGPU_LS + 1] - a.kcol_lofs[TGPU_LS];
+        const size_t row_bytes = (size_t)W * sizeof(double);
+        const int fit = a.kcol_map && W > 0 && budget > kStatic ? (int)((budget - kStatic) / row_bytes) : 0;
         const int nkr = fit < NP + NQ ? fit : NP + NQ;
-        if (a.n_ket_avg >= a.krow_min_kets && nkr > 0 && 2 * nkr >= NP + NQ) {
+        // (no K digested, as in the UHF ket-side J pass: no rows)
+        if (a.kp != HUGE_VAL && a.n_ket_avg >= a.krow_min_kets && nkr > 0 && 2 * nkr >= NP + NQ) {
             JKArgs b = a;
             b.krow_rows = nkr;
             const size_t rows = (size_t)nkr * row_bytes;
@@ -54,10 +55,6 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(b);
             return 0;
         }
-#else
-        (void)NP;
-        (void)NQ;
-#endif
         const int blocks = (a.n_bra + 3) / 4;
         if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, false><<<blocks, 256, 0, s>>>(a);
         else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, false><<<blocks, 256, 0, s>>>(a);

@@ -332,10 +332,19 @@ struct JKArgs {
     int n_ket_avg;  // ket-bin pairs of a single molecule (0 for batches)
     double *ws;    // wavefront-form workspace of ws_doubles doubles: a class uses
     long long ws_doubles;  // min(8192, ws_doubles / WaveCfg::WS) slots
-    int max_nbf;   // largest molecule (basis functions): the LDS rows of the KROWS form
+    int max_nbf;   // largest molecule (basis functions)
     int krow_rows;   // KROWS form: the first krow_rows K rows of the bra pair live in LDS
     int krow_budget; // cap on a work-group's LDS in the KROWS form (bytes; 0: default)
     int krow_min_kets;  // shortest ket segment (n_ket_avg) for the KROWS form
+    // KROWS form, single molecules: the K columns a class pair touches are the functions
+    // of its ket shells' two angular momenta, so an LDS row holds only those, numbered
+    // within each momentum.  kcol[pair] = pack16(number of the pair's first function
+    // among the functions of its first shell's momentum, the same for its second shell);
+    // the functions of momentum l are kcol_map[kcol_lofs[l] .. kcol_lofs[l + 1]).  Null
+    // for batches.
+    const int *kcol;
+    const int *kcol_map;
+    int kcol_lofs[TGPU_LMAX + 2];
     // UHF pass: J from Tonto's ket side only, X(cd) += f (ab|cd) P(ab) with ab the pair of
     // larger Tonto index, for ab != cd (the updates Tonto's UHF engine puts into JA
     // through JBcd => JA(cd), molecule.fock.foo:2637)
@@ -465,11 +474,13 @@ __device__ __forceinline__ void wave_lds_sync()
 // The J/K kernel for one light class pair (LP LQ | LR LS): one lane per quartet
 // ----------------------------------------------------------------------------------
 // KROWS: one bra pair per work-group, its four wavefronts taking interleaved rounds of the
-// ket prefix; the K updates of the bra pair all fall in its NP + NQ rows, and the first
-// krow_rows of them (all, when n_bf allows) accumulate in LDS (dynamic shared memory,
-// krow_rows x n_bf doubles) and are flushed once, row by row, with coalesced atomics; the
-// other rows are updated with global atomics.  Otherwise four bra pairs per work-group,
-// one per wavefront, and every K update is a global atomic.
+// ket prefix; the K updates of the bra pair all fall in its NP + NQ rows, and there only
+// in the columns of momentum LR and LS.  The first krow_rows rows (all, when the LDS
+// budget allows), restricted to those columns, accumulate in LDS (dynamic shared memory,
+// krow_rows x W doubles, W = the number of functions of momentum LR, plus those of
+// momentum LS if LS != LR) and are flushed once, row by row, with atomics that skip zeros;
+// the other rows are updated with global atomics.  Otherwise four bra pairs per
+// work-group, one per wavefront, and every K update is a global atomic.
 template <int LP, int LQ, int LR, int LS, bool MIXED, bool KROWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves(LP, LQ, LR, LS), 8))) void jk_kernel(JKArgs a)
 {
@@ -499,14 +510,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     load_bra(a, i, b);
     const int ld = b.ld, fp = b.fp, fq = b.fq;
     const int nkr = KROWS ? a.krow_rows : 0;  // K rows accumulated in LDS
+    // LDS row: the nR functions of momentum LR, then (LS != LR) those of momentum LS
+    const int nR = KROWS ? a.kcol_lofs[LR + 1] - a.kcol_lofs[LR] : 0;
+    const int W = KROWS ? (LR == LS ? nR : nR + a.kcol_lofs[LS + 1] - a.kcol_lofs[LS]) : 0;
     if (KROWS) {
-        for (int t = threadIdx.x; t < nkr * ld; t += blockDim.x) krow[t] = 0.0;
+        for (int t = threadIdx.x; t < nkr * W; t += blockDim.x) krow[t] = 0.0;
         __syncthreads();
     }
-    // K(row, col) += v: row = bra function fp + p (p < NP) or fq + q (NP + q)
-    auto kadd = [&](const int row, const int col, const double v) {
+    // K(row, col) += v: row = bra function fp + p (p < NP) or fq + q (NP + q); col the
+    // molecule's function, ccol its LDS column
+    auto kadd = [&](const int row, const int col, const int ccol, const double v) {
         if (KROWS && row < nkr) {
-            __hip_atomic_fetch_add(&krow[row * ld + col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&krow[row * W + ccol], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);
         }
@@ -590,6 +605,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
             // P is symmetric: P(s,q) etc. are read as P(q,s) from the bra pair's rows,
             // which every lane of the wavefront shares (cache-resident), instead of one
             // row per lane
+            // LDS columns of the ket's functions (KROWS)
+            int ccr = 0, ccs = 0;
+            if (KROWS) {
+                const int kc = ldc(a.kcol + t.sk);
+                ccr = hi16(kc);
+                ccs = (LR == LS ? 0 : nR) + lo16(kc);
+            }
             double Psq[NS * NQ], Prq[NR * NQ], Psp[NS * NP], Prp[NR * NP];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
@@ -615,7 +637,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Psq[s * NQ + q];
-                    kadd(p, fr + r, f * acc);
+                    kadd(p, fr + r, ccr + r, f * acc);
                 }
 #pragma unroll
             for (int p = 0; p < NP; ++p)
@@ -627,7 +649,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Prq[r * NQ + q];
-                    kadd(p, fs + s, f * acc);
+                    kadd(p, fs + s, ccs + s, f * acc);
                 }
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -639,7 +661,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Psp[s * NP + p];
-                    kadd(NP + q, fr + r, f * acc);
+                    kadd(NP + q, fr + r, ccr + r, f * acc);
                 }
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
@@ -651,7 +673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
                             acc += I[((p * NQ + q) * NR + r) * NS + s] * Prp[r * NP + p];
-                    kadd(NP + q, fs + s, f * acc);
+                    kadd(NP + q, fs + s, ccs + s, f * acc);
                 }
         }
     };
@@ -724,10 +746,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     if (KROWS) {
         // every wavefront of the work-group is here (one bra pair, no early return)
         __syncthreads();
-        for (int t = threadIdx.x; t < nkr * ld; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nkr * W; t += blockDim.x) {
             const double v = krow[t];
-            const int row = t / ld, col = t - row * ld;
-            if (v != 0.0) unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);
+            const int row = t / W, cc = t - row * W;
+            if (v != 0.0) {
+                const int col = ldc(a.kcol_map + (cc < nR ? a.kcol_lofs[LR] + cc : a.kcol_lofs[LS] + cc - nR));
+                unsafeAtomicAdd(&b.Y[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col], v);
+            }
         }
     }
 }

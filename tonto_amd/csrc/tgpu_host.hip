@@ -354,19 +354,21 @@ static const bool kHeavy[NQCLS] = {
 #undef TGPU_HEAVY_FLAG
 };
 
-// Light class pairs whose wavefront form is the default: their lane-per-quartet kernels
-// need 230-380 registers (one or two wavefronts per SIMD), and the wavefront form was the
-// faster one on (H2O)64 / cc-pVTZ (tgpu_autotune, FP64 build at an SCF density: (pp|pp)
-// 127 vs 171 ms, (ds|pp) 424 vs 570, (ds|ds) 379 vs 475, (fs|ps) 402 vs 458, (fs|ds) 233
-// vs 359).  tgpu_autotune re-decides per molecule.
-static bool wave_by_default(int lp, int lq, int lr, int ls)
+// Default kernel form per class pair.  Single molecules: the lane-per-quartet form for
+// every class pair that has one; with its K rows in LDS (columns of the ket momenta only)
+// it beat the wavefront form on every light class pair of (H2O)64 / cc-pVTZ (tgpu_autotune,
+// FP64 build at an SCF density: (pp|pp) 57 vs 125 ms, (ds|pp) 180 vs 415, (ds|ds) 92 vs
+// 377, (fs|ps) 159 vs 400, (fs|ds) 151 vs 232).  Batches (no LDS rows): the wavefront
+// form for the light class pairs whose lane kernels need 230-380 registers, (pp|pp),
+// (ds|pp), (ds|ds), (fs|ps), (fs|ds).  tgpu_autotune re-decides per molecule.
+static bool wave_by_default_batch(int lp, int lq, int lr, int ls)
 {
     return (lp == 1 && lq == 1 && lr == 1 && ls == 1) || (lp == 2 && lq == 0 && lr == 1 && ls == 1) ||
            (lp == 2 && lq == 0 && lr == 2 && ls == 0) || (lp == 3 && lq == 0 && lr == 1 && ls == 0) ||
            (lp == 3 && lq == 0 && lr == 2 && ls == 0);
 }
-static const bool kWaveDefault[NQCLS] = {
-#define TGPU_WAVE_DEFAULT(LP, LQ, LR, LS, H) (H || wave_by_default(LP, LQ, LR, LS)),
+static const bool kWaveDefaultBatch[NQCLS] = {
+#define TGPU_WAVE_DEFAULT(LP, LQ, LR, LS, H) (H || wave_by_default_batch(LP, LQ, LR, LS)),
     TGPU_FOR_EACH_CLASS(TGPU_WAVE_DEFAULT)
 #undef TGPU_WAVE_DEFAULT
 };
@@ -393,12 +395,17 @@ struct tgpu_ctx {
     std::vector<double> A, AB, maxI;
     std::vector<double> pz, pzi, pK, pPx, pPy, pPz;
     std::vector<int> bin_max_seg;  // [NBIN] largest molecule segment of each bin
+    // single molecule: number of each shell's first function among the functions of its
+    // angular momentum, and those functions grouped by momentum (the lane kernels' LDS
+    // K rows hold only the columns of the ket shells' momenta)
+    std::vector<int> sh_kcol, kcol_map;
+    int kcol_lofs[TGPU_LMAX + 2] = {0};
     bool have_schwarz = false;
     std::vector<double> maxI_mol, maxI_tonto;
     int variant[NQCLS];
     // device
     DBuf<int> d_rec0, d_rec1, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
-        d_atom_off;
+        d_atom_off, d_kcol, d_kcol_map;
     DBuf<long long> d_pair_off, d_mat_off;
     DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol;
     DBuf<PrimRec> d_prim;
@@ -566,6 +573,13 @@ static int upload_pairs(tgpu_ctx *c)
     std::vector<PrimRec> prim(c->pz.size());
     for (size_t k = 0; k < prim.size(); ++k)
         prim[k] = PrimRec{c->pz[k], c->pzi[k], c->pK[k], c->pPx[k], c->pPy[k], c->pPz[k], 0.0, 0.0};
+    if (c->n_mol == 1) {
+        // per pair: pack16(column number of its first shell's first function, of its
+        // second shell's) within the functions of each shell's momentum
+        std::vector<int> kc(c->tonto.size());
+        for (size_t k = 0; k < kc.size(); ++k) kc[k] = pack16(c->sh_kcol[c->sp[k]], c->sh_kcol[c->sq[k]]);
+        if ((rc = dupload(c, c->d_kcol, kc))) return rc;
+    }
     if ((rc = dupload(c, c->d_rec0, r0)) || (rc = dupload(c, c->d_rec1, r1)) ||
         (rc = dupload(c, c->d_A, c->A)) || (rc = dupload(c, c->d_AB, c->AB)) ||
         (rc = dupload(c, c->d_prim, prim)) || (rc = dupload(c, c->d_seg_start, c->seg_start)) ||
@@ -785,9 +799,25 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
             c->AB[3 * s + t] = c->xyz[3 * c->sh_atom[s0 + p] + t] - c->xyz[3 * c->sh_atom[s0 + q] + t];
         }
     }
+    if (n_mol == 1) {
+        // functions grouped by angular momentum, in shell order within each momentum
+        int cnt[TGPU_LMAX + 1] = {0};
+        c->sh_kcol.resize(c->sh_l.size());
+        for (size_t sh = 0; sh < c->sh_l.size(); ++sh) {
+            c->sh_kcol[sh] = cnt[c->sh_l[sh]];
+            cnt[c->sh_l[sh]] += ncart(c->sh_l[sh]);
+        }
+        c->kcol_lofs[0] = 0;
+        for (int l = 0; l <= TGPU_LMAX; ++l) c->kcol_lofs[l + 1] = c->kcol_lofs[l] + cnt[l];
+        c->kcol_map.assign(c->kcol_lofs[TGPU_LMAX + 1], 0);
+        for (size_t sh = 0; sh < c->sh_l.size(); ++sh)
+            for (int k = 0; k < ncart(c->sh_l[sh]); ++k)
+                c->kcol_map[c->kcol_lofs[c->sh_l[sh]] + c->sh_kcol[sh] + k] = c->sh_bf[sh] + k;
+    }
     build_prims(c);
-    for (int k = 0; k < NQCLS; ++k) c->variant[k] = kWaveDefault[k] ? 1 : 0;
+    for (int k = 0; k < NQCLS; ++k) c->variant[k] = (kHeavy[k] || (n_mol > 1 && kWaveDefaultBatch[k])) ? 1 : 0;
     int rc;
+    if (n_mol == 1 && (rc = dupload(c, c->d_kcol_map, c->kcol_map))) return rc;
     if ((rc = upload_pairs(c))) return rc;
     if ((rc = dupload(c, c->d_sh_l, c->sh_l)) || (rc = dupload(c, c->d_sh_bf, c->sh_bf)) ||
         (rc = dupload(c, c->d_nbf, c->nbf)) || (rc = dupload(c, c->d_shell_off, c->shell_off)) ||
@@ -818,6 +848,7 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_rec0); dfree(c->d_rec1); dfree(c->d_sh_l); dfree(c->d_sh_bf);
     dfree(c->d_seg_start); dfree(c->d_seg_count); dfree(c->d_nbf); dfree(c->d_shell_off);
     dfree(c->d_atom_off); dfree(c->d_pair_off); dfree(c->d_mat_off);
+    dfree(c->d_kcol); dfree(c->d_kcol_map);
     dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_prim);
     dfree(c->d_X); dfree(c->d_Y);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
@@ -1102,6 +1133,9 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     a.krow_rows = 0;
     a.krow_budget = c->krow_budget;
     a.krow_min_kets = c->krow_min_kets;
+    a.kcol = c->n_mol == 1 ? c->d_kcol.p : nullptr;
+    a.kcol_map = c->n_mol == 1 ? c->d_kcol_map.p : nullptr;
+    for (int l = 0; l <= TGPU_LMAX + 1; ++l) a.kcol_lofs[l] = c->kcol_lofs[l];
     const bool prof = rec_end && c->profiling;
     for (int k = 0; k < NQCLS; ++k) {
         const int CX = kClassPair[k][0], CY = kClassPair[k][1];

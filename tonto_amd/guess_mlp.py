@@ -80,6 +80,38 @@ def cart_transform(l, Q):
     return T
 
 
+def cart_transform_batch(l, Q):
+    """cart_transform for a stack of frames Q (nb, 3, 3): (nb, n, n)."""
+    Q = np.asarray(Q, np.float64)
+    nb = Q.shape[0]
+    n = (l + 1) * (l + 2) // 2
+    if l == 0:
+        return np.ones((nb, 1, 1))
+    seqs, mono, axes = _multi(l)
+    onehot = np.zeros((seqs.shape[0], n))
+    onehot[np.arange(seqs.shape[0]), mono] = 1.0
+    T = np.empty((nb, n, n))
+    for i, ax in enumerate(axes):
+        prod = np.ones((nb, seqs.shape[0]))
+        for pos in range(l):
+            prod = prod * Q[:, seqs[:, pos], ax[pos]]
+        T[:, i, :] = prod @ onehot
+    return T
+
+
+def atom_transform_batch(ls, Q):
+    """atom_transform for a stack of frames Q (nb, 3, 3)."""
+    blocks = [cart_transform_batch(l, Q) for l in ls]
+    n = sum(b.shape[1] for b in blocks)
+    T = np.zeros((Q.shape[0], n, n))
+    o = 0
+    for b in blocks:
+        k = b.shape[1]
+        T[:, o:o + k, o:o + k] = b
+        o += k
+    return T
+
+
 def atom_transform(ls, Q):
     """Block-diagonal transform of one atom's shells (angular momenta ls)."""
     blocks = [cart_transform(l, Q) for l in ls]
@@ -176,6 +208,113 @@ class Cluster:
         x[N_RBF + 1 + k:] = self._env(b, Q, a)
         return x
 
+    # ------------------------------------------------------------ whole-cluster forms
+    # The same frames and descriptors as frame() and features(), for every block at once
+    # (numpy over blocks; the environment sums are formed once per atom and turned into
+    # each block's frame, which is linear in the directions).
+
+    def _mate_refs(self):
+        """per atom: the on-site frame's z vector and reference vector (zero if none)"""
+        n = len(self.symbols)
+        ez = np.zeros((n, 3))
+        ref = np.zeros((n, 3))
+        for a in range(n):
+            mates = self._mates(a)
+            hs = [m for m in mates if not self.is_O[m]]
+            if self.is_O[a]:
+                ez[a] = self.bond_sum[a]
+                if len(hs) >= 2:
+                    ref[a] = self.xyz[hs[0]] - self.xyz[hs[1]]
+            else:
+                o = [m for m in mates if self.is_O[m]]
+                ez[a] = self.xyz[o[0]] - self.xyz[a]
+                if hs:
+                    ref[a] = self.xyz[hs[0]] - self.xyz[a]
+        return ez, ref
+
+    def frames(self, a, b):
+        """frame(a[i], b[i]) for index arrays a, b: (nb, 3, 3)."""
+        a = np.asarray(a)
+        b = np.asarray(b)
+        nb = a.size
+        on = a == b
+        ez_on, ref_on = self._mate_refs()
+        ez = np.where(on[:, None], ez_on[a], self.xyz[b] - self.xyz[a])
+        ez = ez / np.linalg.norm(ez, axis=1, keepdims=True)
+        # candidates in frame()'s order: on-site (ref, x, y, -), pair (bond_sum a, b, x, y)
+        c0 = np.where(on[:, None], ref_on[a], self.bond_sum[a])
+        c1 = np.where(on[:, None], np.array([1.0, 0, 0]), self.bond_sum[b])
+        c2 = np.where(on[:, None], np.array([0, 1.0, 0]), np.array([1.0, 0, 0]))
+        c3 = np.where(on[:, None], np.zeros(3), np.array([0, 1.0, 0]))
+        C = np.stack([c0, c1, c2, c3], axis=1)                           # (nb, 4, 3)
+        perp = C - np.einsum("nkc,nc->nk", C, ez)[:, :, None] * ez[:, None, :]
+        nrm = np.linalg.norm(perp, axis=2)
+        ok = nrm > 0.1 * np.maximum(np.linalg.norm(C, axis=2), 1e-12)
+        first = np.argmax(ok, axis=1)
+        ex = perp[np.arange(nb), first] / nrm[np.arange(nb), first][:, None]
+        ey = np.cross(ez, ex)
+        return np.stack([ex, ey, ez], axis=1)
+
+    def _env_sums(self):
+        """per atom and element of the neighbour: sum of g_k(r) and of g_k(r) u (global
+        axes) over neighbours within R_CUT: (n, 2, N_RBF), (n, 2, N_RBF, 3).  Formed once
+        per cluster."""
+        if getattr(self, "_sums", None) is not None:
+            return self._sums
+        n = len(self.symbols)
+        S0 = np.zeros((n, 2, N_RBF))
+        S1 = np.zeros((n, 2, N_RBF, 3))
+        for a in range(n):
+            near = np.nonzero(self.D[a] < R_CUT)[0]
+            near = near[near != a]
+            if near.size:
+                r = self.D[a, near]
+                u = (self.xyz[near] - self.xyz[a]) / r[:, None]
+                g = rbf(r)
+                for e, mask in ((0, self.is_O[near]), (1, ~self.is_O[near])):
+                    S0[a, e] = g[mask].sum(axis=0)
+                    S1[a, e] = g[mask].T @ u[mask]
+        self._sums = (S0, S1)
+        return S0, S1
+
+    def features_batch(self, a, b, Q):
+        """features(a[i], b[i], Q[i]) for every block: (nb, N_IN)."""
+        a = np.asarray(a)
+        b = np.asarray(b)
+        S0, S1 = self._env_sums()
+        r = self.D[a, b]
+        X = np.empty((a.size, N_IN))
+        X[:, :N_RBF] = rbf(r)
+        X[:, N_RBF] = (self.mol[a] == self.mol[b]).astype(np.float64)
+        k = 4 * N_RBF * 2
+        for col, (x, y) in ((N_RBF + 1, (a, b)), (N_RBF + 1 + k, (b, a))):
+            # environment of x without y (y is a neighbour of x unless y == x)
+            s0 = S0[x].copy()
+            s1 = S1[x].copy()
+            drop = (y != x) & (self.D[x, y] < R_CUT)
+            if drop.any():
+                i = np.nonzero(drop)[0]
+                ry = self.D[x[i], y[i]]
+                gy = rbf(ry)
+                uy = (self.xyz[y[i]] - self.xyz[x[i]]) / ry[:, None]
+                ey = np.where(self.is_O[y[i]], 0, 1)
+                s0[i, ey] -= gy
+                s1[i, ey] -= gy[:, :, None] * uy[:, None, :]
+            f = np.empty((a.size, 2, N_RBF, 4))
+            f[..., 0] = s0
+            f[..., 1:] = (s1.reshape(a.size, 2 * N_RBF, 3) @ np.swapaxes(Q, 1, 2)).reshape(a.size, 2, N_RBF, 3)
+            X[:, col:col + k] = f.reshape(a.size, k)
+        return X
+
+    def block_index(self):
+        """type -> (a, b) index arrays of its blocks, in blocks() order."""
+        out = {}
+        for t, a, b in self.blocks():
+            out.setdefault(t, ([], []))
+            out[t][0].append(a)
+            out[t][1].append(b)
+        return {t: (np.array(x), np.array(y)) for t, (x, y) in out.items()}
+
     def blocks(self):
         """[(type, a, b)] of every predicted block."""
         out = []
@@ -234,21 +373,34 @@ def out_dim(t, nbf_O, nbf_H):
     return n[t[0]] * n[t[1]]
 
 
+def _gather_blocks(P, first, nbf, a, b, na, nb):
+    """P's (na x nb) blocks at atoms a[i], b[i]: (n, na, nb)."""
+    ri = np.asarray(first)[a][:, None] + np.arange(na)[None, :]
+    ci = np.asarray(first)[b][:, None] + np.arange(nb)[None, :]
+    return P[ri[:, :, None], ci[:, None, :]]
+
+
 def training_rows(mol, P, atom_density):
     """(type -> (features, targets)) of one converged density P (numpy, Tonto order).
     atom_density: symbol -> promolecule block of that atom (numpy)."""
     cl = Cluster.from_molecule(mol)
     first, ls, nbf = atom_layout(mol)
-    rows = {t: ([], []) for t in TYPES}
-    for t, a, b in cl.blocks():
-        Q = cl.frame(a, b)
-        blk = P[first[a]:first[a] + nbf[a], first[b]:first[b] + nbf[b]]
-        loc = atom_transform(ls[a], Q).T @ blk @ atom_transform(ls[b], Q)
-        if a == b:
-            loc = loc - atom_density[mol.symbols[a]]
-        rows[t][0].append(cl.features(a, b, Q))
-        rows[t][1].append(_pack(t, loc))
-    return {t: (np.array(x), np.array(y)) for t, (x, y) in rows.items() if x}
+    rows = {}
+    for t, (a, b) in cl.block_index().items():
+        Q = cl.frames(a, b)
+        na, nb = nbf[a[0]], nbf[b[0]]
+        blk = _gather_blocks(P, first, nbf, a, b, na, nb)
+        TA = atom_transform_batch(ls[a[0]], Q)
+        TB = atom_transform_batch(ls[b[0]], Q)
+        loc = np.swapaxes(TA, 1, 2) @ blk @ TB
+        if t in ("O", "H"):
+            loc = loc - atom_density[mol.symbols[a[0]]][None]
+            iu = np.triu_indices(na)
+            Y = loc[:, iu[0], iu[1]]
+        else:
+            Y = loc.reshape(len(a), -1)
+        rows[t] = (cl.features_batch(a, b, Q), Y)
+    return rows
 
 
 def assemble(mol, atom_density, predict):
@@ -257,20 +409,28 @@ def assemble(mol, atom_density, predict):
     cl = Cluster.from_molecule(mol)
     first, ls, nbf = atom_layout(mol)
     P = np.zeros((mol.n_bf, mol.n_bf))
-    by_type = {}
-    for t, a, b in cl.blocks():
-        Q = cl.frame(a, b)
-        by_type.setdefault(t, []).append((a, b, Q, cl.features(a, b, Q)))
-    for t, items in by_type.items():
-        Y = predict(t, np.array([it[3] for it in items]))
-        for (a, b, Q, _), y in zip(items, Y):
-            loc = _unpack(t, y, nbf[a], nbf[b])
-            if a == b:
-                loc = loc + atom_density[mol.symbols[a]]
-            blk = atom_transform(ls[a], Q.T).T @ loc @ atom_transform(ls[b], Q.T)
-            P[first[a]:first[a] + nbf[a], first[b]:first[b] + nbf[b]] = blk
-            if a != b:
-                P[first[b]:first[b] + nbf[b], first[a]:first[a] + nbf[a]] = blk.T
+    for t, (a, b) in cl.block_index().items():
+        Q = cl.frames(a, b)
+        Y = predict(t, cl.features_batch(a, b, Q))
+        na, nb = nbf[a[0]], nbf[b[0]]
+        if t in ("O", "H"):
+            loc = np.zeros((len(a), na, na))
+            iu = np.triu_indices(na)
+            loc[:, iu[0], iu[1]] = Y
+            loc = loc + np.swapaxes(loc, 1, 2)
+            loc[:, np.arange(na), np.arange(na)] *= 0.5
+            loc = loc + atom_density[mol.symbols[a[0]]][None]
+        else:
+            loc = np.asarray(Y).reshape(len(a), na, nb)
+        QT = np.swapaxes(Q, 1, 2)
+        TA = atom_transform_batch(ls[a[0]], QT)
+        TB = atom_transform_batch(ls[b[0]], QT)
+        blk = np.swapaxes(TA, 1, 2) @ loc @ TB
+        ri = np.asarray(first)[a][:, None] + np.arange(na)[None, :]
+        ci = np.asarray(first)[b][:, None] + np.arange(nb)[None, :]
+        P[ri[:, :, None], ci[:, None, :]] = blk
+        if t not in ("O", "H"):
+            P[ci[:, :, None], ri[:, None, :]] = np.swapaxes(blk, 1, 2)
     return P
 
 
@@ -329,10 +489,12 @@ def default_model():
 
 
 def gpu_forward(model, device="cuda"):
-    """forward(type, params, Xn) on the GPU through the tmlp kernels (one DeviceNet per type)."""
+    """forward(type, params, Xn) on the GPU through the tmlp kernels (one DeviceNet per type,
+    uploaded once per model and device)."""
     import torch
     from .mlp import DeviceNet
-    nets = {}
+    cache = model.__dict__.setdefault("_device_nets", {})
+    nets = cache.setdefault(str(device), {})
 
     def forward(t, params, Xn):
         if t not in nets:

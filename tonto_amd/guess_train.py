@@ -35,34 +35,63 @@ def _log(*a):
     print(*a, flush=True)
 
 
-def generate(n_clusters, nmin, nmax, seed0, log=_log):
+def _generate_part(args):
+    """Training rows of the clusters `idx` (indices into the seeded size list), in a
+    worker process of its own: (rows, SCF iterations, skipped indices)."""
+    idx, sizes, seed0 = args
     import torch
     from .driver import GPUSCF
     from .geometry import water_cluster
-    rng = np.random.default_rng(seed0)
-    sizes = rng.integers(nmin, nmax + 1, n_clusters)
     rows = {t: ([], []) for t in G.TYPES}
-    t0 = time.perf_counter()
-    its = []
+    its, skipped = [], []
     atom = None
-    for i, n in enumerate(sizes):
-        mol = water_cluster(int(n), BASIS, seed=seed0 + i)
+    for i in idx:
+        mol = water_cluster(int(sizes[i]), BASIS, seed=seed0 + int(i))
         s = GPUSCF(mol)
         if atom is None:
             atom = {sym: s.atom_density(sym) for sym in ("O", "H")}
         r = s.run(guess="promolecule", convergence=1e-8)
         if not r.converged:
-            log(f"cluster {i} (n={n}) did not converge; skipped")
+            skipped.append(int(i))
             continue
         its.append(r.iterations)
         for t, (x, y) in G.training_rows(mol, r.P.cpu().numpy(), atom).items():
             rows[t][0].append(x.astype(np.float32))
             rows[t][1].append(y.astype(np.float32))
-        if (i + 1) % 25 == 0 or i + 1 == n_clusters:
-            log(f"data {i + 1}/{n_clusters} clusters {time.perf_counter() - t0:.1f}s "
-                f"(mean SCF iterations {np.mean(its):.1f})")
         del s
-        torch.cuda.empty_cache() if (i + 1) % 200 == 0 else None
+    torch.cuda.synchronize()
+    out = {t: (np.concatenate(x), np.concatenate(y)) for t, (x, y) in rows.items() if x}
+    return out, its, skipped, atom
+
+
+def generate(n_clusters, nmin, nmax, seed0, workers=1, log=_log):
+    """Converged densities of n_clusters seeded clusters turned into training rows.  With
+    workers > 1 the clusters are dealt out to that many processes sharing the GPU (each
+    cluster's data depends on its seed only, so the rows are the same; they are merged
+    in cluster order per worker)."""
+    rng = np.random.default_rng(seed0)
+    sizes = rng.integers(nmin, nmax + 1, n_clusters)
+    t0 = time.perf_counter()
+    parts = [np.arange(w, n_clusters, workers) for w in range(workers)]
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(workers) as pool:
+            res = pool.map(_generate_part, [(p, sizes, seed0) for p in parts])
+    else:
+        res = [_generate_part((parts[0], sizes, seed0))]
+    rows = {t: ([], []) for t in G.TYPES}
+    its, skipped, atom = [], [], None
+    for out, it, sk, at in res:
+        its += it
+        skipped += sk
+        atom = atom or at
+        for t, (x, y) in out.items():
+            rows[t][0].append(x)
+            rows[t][1].append(y)
+    for i in skipped:
+        log(f"cluster {i} (n={sizes[i]}) did not converge; skipped")
+    log(f"data {n_clusters} clusters on {workers} worker(s) {time.perf_counter() - t0:.1f}s "
+        f"(mean SCF iterations {np.mean(its):.1f})")
     data = {t: (np.concatenate(x), np.concatenate(y)) for t, (x, y) in rows.items() if x}
     return data, atom, sizes, time.perf_counter() - t0
 
@@ -111,7 +140,7 @@ def train_net(t, X, Y, epochs, batch, lr, seed, log=_log):
 
 def train(args):
     t0 = time.perf_counter()
-    data, atom, sizes, t_data = generate(args.clusters, args.min, args.max, args.seed0)
+    data, atom, sizes, t_data = generate(args.clusters, args.min, args.max, args.seed0, args.workers)
     _log("rows: " + ", ".join(f"{t} {X.shape[0]}x{X.shape[1]}->{Y.shape[1]}" for t, (X, Y) in data.items()))
     nets, val = {}, {}
     t1 = time.perf_counter()
@@ -129,8 +158,10 @@ def train(args):
 
 
 def evaluate(n_clusters, n_waters=20, seed0=EVAL_SEED0, model=None, log=_log):
-    """SCF iterations and wall time from the promolecule and the learned guess on held-out
-    clusters; the converged energies must agree (the guess only moves the iteration count)."""
+    """SCF iterations and wall time (making the guess included) from the promolecule and
+    the learned guess on held-out clusters, after a warm-up shared by both and in
+    alternating order; the converged energies must agree (the guess only moves the
+    iteration count)."""
     import torch
     from .driver import GPUSCF
     from .geometry import water_cluster
@@ -142,14 +173,22 @@ def evaluate(n_clusters, n_waters=20, seed0=EVAL_SEED0, model=None, log=_log):
     for i in range(n_clusters):
         mol = water_cluster(n_waters, BASIS, seed=seed0 + i)
         s = GPUSCF(mol)
+        # warm-up shared by both runs: atomic densities, the build's kernels, the network
         s.atom_density("O"), s.atom_density("H")
-        r0 = s.run(guess="promolecule")
+        s.jk(s.guess(), "full")
+        s.guess("mlp", model)
         torch.cuda.synchronize()
         t = time.perf_counter()
         s.guess("mlp", model)
         torch.cuda.synchronize()
         tg = time.perf_counter() - t
-        r1 = s.run(guess="mlp", guess_model=model)
+        # the two runs in alternating order
+        if i % 2 == 0:
+            r0 = s.run(guess="promolecule")
+            r1 = s.run(guess="mlp", guess_model=model)
+        else:
+            r1 = s.run(guess="mlp", guess_model=model)
+            r0 = s.run(guess="promolecule")
         out["sad_iterations"].append(r0.iterations)
         out["mlp_iterations"].append(r1.iterations)
         out["sad_wall"].append(r0.wall)
@@ -174,6 +213,8 @@ def main(argv=None):
     a.add_argument("--batch", type=int, default=2048)
     a.add_argument("--lr", type=float, default=2e-3)
     a.add_argument("--out", default=G.DEFAULT_MODEL)
+    a.add_argument("--workers", type=int, default=4,
+                   help="processes generating the data (sharing the GPU)")
     e = sub.add_parser("eval")
     e.add_argument("--clusters", type=int, default=200)
     e.add_argument("--waters", type=int, default=20)
