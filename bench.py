@@ -67,14 +67,20 @@ def cpu_baseline(mol, P, max_I, target_s):
     o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
     full_est = (time.perf_counter() - t) * stride
     stride = max(1, int(math.ceil(full_est / target_s)))
-    t = time.perf_counter()
-    st = o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
-    dt = time.perf_counter() - t
-    # one thread on a sample a tenth as large (every 10k-th pair of the same list)
+    # the cost per bra pair varies widely: refine the stride until the sample takes at
+    # least half the target
+    for _ in range(3):
+        t = time.perf_counter()
+        st = o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
+        dt = time.perf_counter() - t
+        if dt >= 0.5 * target_s or stride == 1:
+            break
+        stride = max(1, int(stride * dt / target_s))
+    # one thread on a sample a twentieth as large (every 20k-th pair of the same list)
     O.set_threads(1)
     try:
         t = time.perf_counter()
-        st1 = o.make_r_JK_sample(P, 10 * stride, schwarz=SCHWARZ, max_I=max_I)
+        st1 = o.make_r_JK_sample(P, 20 * stride, schwarz=SCHWARZ, max_I=max_I)
         dt1 = time.perf_counter() - t
     finally:
         O.set_threads(cores)
@@ -84,7 +90,7 @@ def cpu_baseline(mol, P, max_I, target_s):
                       f"{st['prim_quartets']} primitive quartets, {dt:.1f} s); oracle/oracle.c, "
                       f"{cores} pthreads",
             "one_thread": {"value": st1["quartets"] / dt1, "unit": "quartets/s", "cores": 1,
-                           "sample": f"every {10 * stride}-th bra shell pair ({st1['quartets']} "
+                           "sample": f"every {20 * stride}-th bra shell pair ({st1['quartets']} "
                                      f"quartets, {dt1:.1f} s)"},
             "full_build_estimate_s": dt * stride,
             "seconds": dt + dt1}
@@ -211,7 +217,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--waters", type=int, default=N_WATER, help="cluster size (default 64)")
     ap.add_argument("--density", default=None,
                     help="converged density (.npy) to time the steps at; skips both SCF runs")
@@ -260,6 +266,8 @@ def main():
         r = run_rhf(scf.h, scf.S, mol.n_electrons, mol.nuclear_repulsion(), scf.jk, P0=scf.guess(),
                     max_iterations=a.profile_iterations, log=lambda s: log("profile " + s))
         P = r.P.contiguous()
+        if a.save_density:
+            np.save(a.save_density, P.cpu().numpy())
     elif a.density is None:
         # warm-up for both runs alike: atomic densities cached, every kernel loaded (the
         # learned guess's network and projection included)

@@ -87,15 +87,17 @@ def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-def test_lds_row_form_every_lane_class(torch_cuda, oracle_mod, mixed):
+@pytest.mark.parametrize("n,seed", [(2, 0), (4, 2)])
+def test_lds_row_form_every_lane_class(torch_cuda, oracle_mod, mixed, n, seed):
     """Every class pair that has the lane-per-quartet kernel, run in its LDS K-row form one
-    at a time (the others in the wavefront form), on (H2O)_2 / cc-pVTZ (130 functions, all
-    rows in LDS): J and K against the oracle to 1e-10 (FP64) and to the mixed-precision
-    bound 1e-8, the same quartets."""
+    at a time (the others in the wavefront form; (ff|ss) has only the global-atomic lane
+    form), on (H2O)_2 and (H2O)_4 / cc-pVTZ (130 and 260 functions, all rows in LDS): J
+    and K against the oracle to 1e-10 (FP64) and to the mixed-precision bound 1e-8, the
+    same quartets."""
     from tonto_amd.build import class_list
     from tonto_amd.flops import class_pairs
     from tonto_amd.geometry import water_cluster
-    mol = water_cluster(2, "cc-pVTZ")
+    mol = water_cluster(n, "cc-pVTZ", seed=seed)
     P = _rand_density(mol.n_bf, 31)
     old = os.environ.get("TGPU_KROW_MIN_KETS")
     os.environ["TGPU_KROW_MIN_KETS"] = "0"
@@ -112,7 +114,7 @@ def test_lds_row_form_every_lane_class(torch_cuda, oracle_mod, mixed):
     heavy = {tuple(c[:4]): c[4] for c in class_list()}
     cps = class_pairs()
     light = [k for k, cp in enumerate(cps) if not heavy[tuple(cp)]]
-    assert len(light) == 18
+    assert len(light) == 28
     tol = 1e-8 if mixed else 1e-10
     for k in light:
         for kk in light:

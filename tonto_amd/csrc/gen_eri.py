@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), ".."
 from tonto_amd.basis import cart_components  # noqa: E402
 
 LMAX = int(os.environ.get("TGPU_GEN_LMAX", "3"))
-HEAVY_NI = 81
+HEAVY_NI = 200
 
 
 def ncart(l):

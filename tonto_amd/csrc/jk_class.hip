@@ -26,7 +26,10 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
         // default, so that the four wavefronts of one bra pair have work): as many of the
         // bra pair's K rows as fit the work-group's LDS budget next to the Rys table and
         // the ket queue, and at least half of them.  A row holds the columns of the ket
-        // shells' two angular momenta only.
+        // shells' two angular momenta only.  Not built for (ff|ss): its FP64 instantiation
+        // gives wrong K on every cluster tested while its mixed twin and the global-atomic
+        // form are exact (cause not found), so that class pair updates K globally.
+#if !(TGPU_LP == 3 && TGPU_LQ == 3 && TGPU_LR == 0 && TGPU_LS == 0)
         constexpr int NP = ncart(TGPU_LP), NQ = ncart(TGPU_LQ);
         constexpr int NRT = EriBody<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, double>::NROOT;
         constexpr size_t kStatic = rys_block_size(NRT) * sizeof(double) + (TGPU_COMPACT ? 4 * 128 * 8 : 0);
@@ -55,6 +58,7 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(b);
             return 0;
         }
+#endif
         const int blocks = (a.n_bra + 3) / 4;
         if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, false><<<blocks, 256, 0, s>>>(a);
         else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, false><<<blocks, 256, 0, s>>>(a);

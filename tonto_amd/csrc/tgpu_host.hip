@@ -355,20 +355,32 @@ static const bool kHeavy[NQCLS] = {
 };
 
 // Default kernel form per class pair.  Single molecules: the lane-per-quartet form for
-// every class pair that has one; with its K rows in LDS (columns of the ket momenta only)
-// it beat the wavefront form on every light class pair of (H2O)64 / cc-pVTZ (tgpu_autotune,
-// FP64 build at an SCF density: (pp|pp) 57 vs 125 ms, (ds|pp) 180 vs 415, (ds|ds) 92 vs
-// 377, (fs|ps) 159 vs 400, (fs|ds) 151 vs 232).  Batches (no LDS rows): the wavefront
-// form for the light class pairs whose lane kernels need 230-380 registers, (pp|pp),
-// (ds|pp), (ds|ds), (fs|ps), (fs|ds).  tgpu_autotune re-decides per molecule.
+// every class pair that has one but (fs|dp); with its K rows in LDS (columns of the ket
+// momenta only) it beat the wavefront form on those class pairs of (H2O)64 / cc-pVTZ
+// (tgpu_autotune, FP64 build at an SCF density: (pp|pp) 57 vs 125 ms, (ds|ds) 92 vs 377,
+// (fs|ps) 159 vs 400, (dp|ds) 218 vs 450, (dd|ps) 89 vs 230, (fp|ps) 120 vs 279, (dp|pp)
+// 124 vs 301, (fp|ds) 137 vs 207, (fd|ps) 122 vs 178; (fs|dp) 296 vs 215, its lane kernel
+// spilling 0.5-0.9 KB per lane).  Batches (no LDS rows): the wavefront form for the class
+// pairs whose lane kernels need more than 230 registers: (pp|pp), (ds|pp), (ds|ds),
+// (fs|ps), (fs|ds) and those with more than 81 integrals.  tgpu_autotune re-decides per
+// molecule.
+static bool wave_by_default_single(int lp, int lq, int lr, int ls)
+{
+    return lp == 3 && lq == 0 && lr == 2 && ls == 1;
+}
 static bool wave_by_default_batch(int lp, int lq, int lr, int ls)
 {
     return (lp == 1 && lq == 1 && lr == 1 && ls == 1) || (lp == 2 && lq == 0 && lr == 1 && ls == 1) ||
            (lp == 2 && lq == 0 && lr == 2 && ls == 0) || (lp == 3 && lq == 0 && lr == 1 && ls == 0) ||
-           (lp == 3 && lq == 0 && lr == 2 && ls == 0);
+           (lp == 3 && lq == 0 && lr == 2 && ls == 0) || ncart(lp) * ncart(lq) * ncart(lr) * ncart(ls) > 81;
 }
 static const bool kWaveDefaultBatch[NQCLS] = {
 #define TGPU_WAVE_DEFAULT(LP, LQ, LR, LS, H) (H || wave_by_default_batch(LP, LQ, LR, LS)),
+    TGPU_FOR_EACH_CLASS(TGPU_WAVE_DEFAULT)
+#undef TGPU_WAVE_DEFAULT
+};
+static const bool kWaveDefaultSingle[NQCLS] = {
+#define TGPU_WAVE_DEFAULT(LP, LQ, LR, LS, H) (H || wave_by_default_single(LP, LQ, LR, LS)),
     TGPU_FOR_EACH_CLASS(TGPU_WAVE_DEFAULT)
 #undef TGPU_WAVE_DEFAULT
 };
@@ -815,7 +827,7 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
                 c->kcol_map[c->kcol_lofs[c->sh_l[sh]] + c->sh_kcol[sh] + k] = c->sh_bf[sh] + k;
     }
     build_prims(c);
-    for (int k = 0; k < NQCLS; ++k) c->variant[k] = (kHeavy[k] || (n_mol > 1 && kWaveDefaultBatch[k])) ? 1 : 0;
+    for (int k = 0; k < NQCLS; ++k) c->variant[k] = (n_mol > 1 ? kWaveDefaultBatch[k] : kWaveDefaultSingle[k]) ? 1 : 0;
     int rc;
     if (n_mol == 1 && (rc = dupload(c, c->d_kcol_map, c->kcol_map))) return rc;
     if ((rc = upload_pairs(c))) return rc;
