@@ -214,13 +214,18 @@ __global__ void cart_to_sph_kernel(const double *__restrict__ Jc, double *__rest
     }
 }
 
-// One-electron S, T, V for one Tonto shell pair per thread (molecule.ints.foo:34-415):
-// Obara-Saika 1-D overlap/kinetic, Rys quadrature with X = p|PC|^2 for the nuclear term.
-__device__ void os1d(double PA, double PB, double hp, int la, int lb, double S[6][6])
+// One-electron S, T, V (molecule.ints.foo:34-415), one shell pair per thread, one launch
+// per class (LA >= LB) so that the 1-D tables and the pair's blocks have compile-time
+// sizes and stay in registers: Obara-Saika 1-D overlap/kinetic, Rys quadrature with
+// X = p|PC|^2 for the nuclear term.
+template <int LA, int LB>
+__device__ __forceinline__ void os1d(double PA, double PB, double hp, double (&S)[LA + 2][LB + 2])
 {
     S[0][0] = 1.0;
-    for (int i = 0; i <= la; ++i)
-        for (int j = 0; j <= lb; ++j) {
+#pragma unroll
+    for (int i = 0; i <= LA + 1; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB + 1; ++j) {
             if (i == 0 && j == 0) continue;
             if (i > 0)
                 S[i][j] = PA * S[i - 1][j] +
@@ -230,56 +235,70 @@ __device__ void os1d(double PA, double PB, double hp, int la, int lb, double S[6
         }
 }
 
+// Cartesian exponents of component i of a shell of angular momentum l (Tonto's order,
+// gaussian_data.foo:295-479) for axis t
+__host__ __device__ constexpr int cart_exp(int l, int i, int t)
+{
+    constexpr signed char tab[4][10][3] = {
+        {{0, 0, 0}},
+        {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+        {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}},
+        {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {1, 1, 1}}};
+    return tab[l][i][t];
+}
+
 struct OneEArgs {
     BatchArgs b;
-    const int *sh_atom, *sh_off, *sh_np, *cx, *cy, *cz;  // cx..: component tables [l][i]
+    const int *sh_atom, *sh_off, *sh_np;
     const double *exps, *coefs, *xyz, *Z;
     double *S, *T, *V;
 };
 
-__global__ void one_e_kernel(OneEArgs a)
+// pairs: (first shell, second shell, molecule) with l(first) = LA, l(second) = LB
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void one_e_kernel(OneEArgs a, const int4 *__restrict__ pairs, int n)
 {
-    const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (g >= a.b.pair_off[a.b.n_mol]) return;
-    const int m = mol_of(a.b.pair_off, a.b.n_mol, g);
-    int x, y;
-    decode_pair(g - a.b.pair_off[m], x, y);
-    x += a.b.shell_off[m];
-    y += a.b.shell_off[m];
+    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB, NR = L / 2 + 1;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int4 pr = pairs[g];
+    const int x = pr.x, y = pr.y, m = pr.z;
     const int nbf = a.b.nbf[m];
     const size_t mo = a.b.mat_off[m];
-    const int la = a.b.sh_l[x], lb = a.b.sh_l[y], na = ncart(la), nb = ncart(lb);
     const double *A = a.xyz + 3 * a.sh_atom[x], *B = a.xyz + 3 * a.sh_atom[y];
-    double sb[10][10], tb[10][10], vb[10][10];
-    for (int i = 0; i < 10; ++i)
-        for (int j = 0; j < 10; ++j) sb[i][j] = tb[i][j] = vb[i][j] = 0.0;
+    double sb[NA][NB], tb[NA][NB], vb[NA][NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) sb[i][j] = tb[i][j] = vb[i][j] = 0.0;
     double AB2 = 0.0;
     for (int t = 0; t < 3; ++t) AB2 += (A[t] - B[t]) * (A[t] - B[t]);
-    const int L = la + lb, nr = L / 2 + 1;
     for (int ia = 0; ia < a.sh_np[x]; ++ia)
         for (int ib = 0; ib < a.sh_np[y]; ++ib) {
             const double al = a.exps[a.sh_off[x] + ia], be = a.exps[a.sh_off[y] + ib];
             const double cc = a.coefs[a.sh_off[x] + ia] * a.coefs[a.sh_off[y] + ib];
             const double p = al + be, mu = al * be / p;
-            double Pc[3], Sx[3][6][6];
+            double Pc[3], Sx[3][LA + 2][LB + 2];
+#pragma unroll
             for (int t = 0; t < 3; ++t) {
                 Pc[t] = (al * A[t] + be * B[t]) / p;
-                os1d(Pc[t] - A[t], Pc[t] - B[t], 0.5 / p, la + 1, lb + 1, Sx[t]);
+                os1d<LA, LB>(Pc[t] - A[t], Pc[t] - B[t], 0.5 / p, Sx[t]);
             }
             const double ek = exp(-mu * AB2);
             const double pre = cc * ek * pow(M_PI / p, 1.5);
-            for (int i = 0; i < na; ++i)
-                for (int j = 0; j < nb; ++j) {
-                    const int xa[3] = {a.cx[la * 10 + i], a.cy[la * 10 + i], a.cz[la * 10 + i]};
-                    const int xb[3] = {a.cx[lb * 10 + j], a.cy[lb * 10 + j], a.cz[lb * 10 + j]};
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
                     double s3[3], t3[3];
+#pragma unroll
                     for (int t = 0; t < 3; ++t) {
-                        const int u = xa[t], v = xb[t];
+                        const int u = cart_exp(LA, i, t), v = cart_exp(LB, j, t);
                         s3[t] = Sx[t][u][v];
                         t3[t] = 2.0 * al * be * Sx[t][u + 1][v + 1] +
-                                (u > 0 && v > 0 ? 0.5 * u * v * Sx[t][u - 1][v - 1] : 0.0) -
-                                (v > 0 ? al * v * Sx[t][u + 1][v - 1] : 0.0) -
-                                (u > 0 ? be * u * Sx[t][u - 1][v + 1] : 0.0);
+                                (u > 0 && v > 0 ? 0.5 * u * v * Sx[t][u > 0 ? u - 1 : 0][v > 0 ? v - 1 : 0] : 0.0) -
+                                (v > 0 ? al * v * Sx[t][u + 1][v > 0 ? v - 1 : 0] : 0.0) -
+                                (u > 0 ? be * u * Sx[t][u > 0 ? u - 1 : 0][v + 1] : 0.0);
                     }
                     sb[i][j] += pre * s3[0] * s3[1] * s3[2];
                     tb[i][j] += pre * (t3[0] * s3[1] * s3[2] + s3[0] * t3[1] * s3[2] + s3[0] * s3[1] * t3[2]);
@@ -288,37 +307,46 @@ __global__ void one_e_kernel(OneEArgs a)
                 const double *Cc = a.xyz + 3 * C;
                 double PC2 = 0.0;
                 for (int t = 0; t < 3; ++t) PC2 += (Pc[t] - Cc[t]) * (Pc[t] - Cc[t]);
-                double u[7], w[7];
-                rys_roots_rt(nr, p * PC2, u, w);
+                double u[NR], w[NR];
+                rys_roots_rt(NR, p * PC2, u, w);
                 const double vpre = -a.Z[C] * 2.0 * M_PI / p * ek * cc;
-                for (int r = 0; r < nr; ++r) {
-                    double I1[3][4][4];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    double I1[3][LA + 1][LB + 1];
+#pragma unroll
                     for (int t = 0; t < 3; ++t) {
                         const double c00 = (Pc[t] - A[t]) + u[r] * (Cc[t] - Pc[t]);
                         const double b10 = (1.0 - u[r]) * 0.5 / p;
-                        double H[7][4];
+                        double H[L + 1][LB + 1];
                         H[0][0] = 1.0;
+#pragma unroll
                         for (int e = 0; e < L; ++e)
-                            H[e + 1][0] = c00 * H[e][0] + (e ? e * b10 * H[e - 1][0] : 0.0);
-                        for (int j = 1; j <= lb; ++j)
-                            for (int i = 0; i <= L - j; ++i)
-                                H[i][j] = H[i + 1][j - 1] + (A[t] - B[t]) * H[i][j - 1];
-                        for (int i = 0; i <= la; ++i)
-                            for (int j = 0; j <= lb; ++j) I1[t][i][j] = H[i][j];
+                            H[e + 1][0] = c00 * H[e][0] + (e ? e * b10 * H[e > 0 ? e - 1 : 0][0] : 0.0);
+#pragma unroll
+                        for (int j = 1; j <= LB; ++j)
+#pragma unroll
+                            for (int i = 0; i <= L - j; ++i) H[i][j] = H[i + 1][j - 1] + (A[t] - B[t]) * H[i][j - 1];
+#pragma unroll
+                        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+                            for (int j = 0; j <= LB; ++j) I1[t][i][j] = H[i][j];
                     }
-                    for (int i = 0; i < na; ++i)
-                        for (int j = 0; j < nb; ++j) {
-                            vb[i][j] += vpre * w[r] *
-                                        I1[0][a.cx[la * 10 + i]][a.cx[lb * 10 + j]] *
-                                        I1[1][a.cy[la * 10 + i]][a.cy[lb * 10 + j]] *
-                                        I1[2][a.cz[la * 10 + i]][a.cz[lb * 10 + j]];
-                        }
+                    const double vw = vpre * w[r];
+#pragma unroll
+                    for (int i = 0; i < NA; ++i)
+#pragma unroll
+                        for (int j = 0; j < NB; ++j)
+                            vb[i][j] += vw * I1[0][cart_exp(LA, i, 0)][cart_exp(LB, j, 0)] *
+                                        I1[1][cart_exp(LA, i, 1)][cart_exp(LB, j, 1)] *
+                                        I1[2][cart_exp(LA, i, 2)][cart_exp(LB, j, 2)];
                 }
             }
         }
     const int fa = a.b.sh_bf[x], fb = a.b.sh_bf[y];
-    for (int i = 0; i < na; ++i)
-        for (int j = 0; j < nb; ++j) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
             const size_t ij = mo + (size_t)(fa + i) * nbf + fb + j, ji = mo + (size_t)(fb + j) * nbf + fa + i;
             a.S[ij] = a.S[ji] = sb[i][j];
             a.T[ij] = a.T[ji] = tb[i][j];
@@ -1526,29 +1554,42 @@ extern "C" int tgpu_one_electron(tgpu_ctx *c, double *S, double *T, double *V)
 {
     if (!c || !S || !T || !V) return TGPU_E_ARG;
     const size_t nn = (size_t)c->n_elem;
-    std::vector<int> cx(4 * 10, 0), cy(4 * 10, 0), cz(4 * 10, 0);
-    // Tonto's Cartesian order (gaussian_data.foo:295-479) for l <= 3
-    static const int tab[4][10][3] = {
-        {{0, 0, 0}},
-        {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
-        {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}},
-        {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {1, 1, 1}}};
-    for (int l = 0; l < 4; ++l)
-        for (int i = 0; i < ncart(l); ++i) {
-            cx[l * 10 + i] = tab[l][i][0];
-            cy[l * 10 + i] = tab[l][i][1];
-            cz[l * 10 + i] = tab[l][i][2];
+    // every shell pair once (Tonto's pairs, b <= a), oriented higher l first and grouped by
+    // class (la, lb): one launch per class
+    std::vector<int4> pairs((size_t)c->n_pairs);
+    std::vector<long long> cls_start(TGPU_NPAIRCLASS + 1, 0);
+    for (int m = 0; m < c->n_mol; ++m) {
+        const int s0 = c->shell_off[m], ns = c->shell_off[m + 1] - s0;
+        for (int a = 0; a < ns; ++a)
+            for (int b = 0; b <= a; ++b) {
+                const int la = c->sh_l[s0 + a], lb = c->sh_l[s0 + b];
+                cls_start[pair_class(std::max(la, lb), std::min(la, lb)) + 1]++;
+            }
+    }
+    for (int k = 0; k < TGPU_NPAIRCLASS; ++k) cls_start[k + 1] += cls_start[k];
+    {
+        std::vector<long long> fill(cls_start.begin(), cls_start.end() - 1);
+        for (int m = 0; m < c->n_mol; ++m) {
+            const int s0 = c->shell_off[m], ns = c->shell_off[m + 1] - s0;
+            for (int a = 0; a < ns; ++a)
+                for (int b = 0; b <= a; ++b) {
+                    const int la = c->sh_l[s0 + a], lb = c->sh_l[s0 + b];
+                    const int x = la >= lb ? s0 + a : s0 + b, y = la >= lb ? s0 + b : s0 + a;
+                    pairs[(size_t)fill[pair_class(std::max(la, lb), std::min(la, lb))]++] = make_int4(x, y, m, 0);
+                }
         }
-    DBuf<int> d_atom, d_off, d_np, d_cx, d_cy, d_cz;
+    }
+    DBuf<int> d_atom, d_off, d_np;
+    DBuf<int4> d_pairs;
     DBuf<double> d_e, d_c, d_xyz, d_Z, d_S, d_T, d_V;
     int rc;
     auto cleanup = [&]() {
-        dfree(d_atom); dfree(d_off); dfree(d_np); dfree(d_cx); dfree(d_cy); dfree(d_cz);
+        dfree(d_atom); dfree(d_off); dfree(d_np); dfree(d_pairs);
         dfree(d_e); dfree(d_c); dfree(d_xyz); dfree(d_Z); dfree(d_S); dfree(d_T); dfree(d_V);
     };
     if ((rc = dupload(c, d_atom, c->sh_atom)) || (rc = dupload(c, d_off, c->sh_off)) ||
-        (rc = dupload(c, d_np, c->sh_np)) || (rc = dupload(c, d_cx, cx)) || (rc = dupload(c, d_cy, cy)) ||
-        (rc = dupload(c, d_cz, cz)) || (rc = dupload(c, d_e, c->exps)) || (rc = dupload(c, d_c, c->coefs)) ||
+        (rc = dupload(c, d_np, c->sh_np)) || (rc = dupload(c, d_pairs, pairs)) ||
+        (rc = dupload(c, d_e, c->exps)) || (rc = dupload(c, d_c, c->coefs)) ||
         (rc = dupload(c, d_xyz, c->xyz)) || (rc = dupload(c, d_Z, c->Z)) || (rc = dalloc(c, d_S, nn)) ||
         (rc = dalloc(c, d_T, nn)) || (rc = dalloc(c, d_V, nn))) {
         cleanup();
@@ -1557,9 +1598,16 @@ extern "C" int tgpu_one_electron(tgpu_ctx *c, double *S, double *T, double *V)
     OneEArgs a;
     a.b = batch_args(c);
     a.sh_atom = d_atom.p; a.sh_off = d_off.p; a.sh_np = d_np.p;
-    a.cx = d_cx.p; a.cy = d_cy.p; a.cz = d_cz.p; a.exps = d_e.p; a.coefs = d_c.p;
+    a.exps = d_e.p; a.coefs = d_c.p;
     a.xyz = d_xyz.p; a.Z = d_Z.p; a.S = d_S.p; a.T = d_T.p; a.V = d_V.p;
-    one_e_kernel<<<(int)((c->n_pairs + 63) / 64), 64, 0, c->stream>>>(a);
+#define TGPU_ONE_E_CASE(LP, LQ, H)                                                              \
+    {                                                                                         \
+        const int k = pair_class(LP, LQ);                                                     \
+        const int n = (int)(cls_start[k + 1] - cls_start[k]);                                 \
+        if (n) one_e_kernel<LP, LQ><<<(n + 63) / 64, 64, 0, c->stream>>>(a, d_pairs.p + cls_start[k], n); \
+    }
+    TGPU_FOR_EACH_PAIRCLASS(TGPU_ONE_E_CASE)
+#undef TGPU_ONE_E_CASE
     hipError_t e = hipGetLastError();
     size_t nout = nn;
     if (c->spherical && e == hipSuccess) {
