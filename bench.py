@@ -54,7 +54,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mol, P, max_I, target_s):
+def cpu_baseline(mol, P, max_I, target_s, schwarz=SCHWARZ):
     """The oracle's FP64 J/K build on a bounded sample of the same build (every k-th bra
     shell pair, k chosen for about target_s seconds on all host cores), at the same
     cutoffs and density; timed on all cores and on one (the same sample)."""
@@ -64,14 +64,14 @@ def cpu_baseline(mol, P, max_I, target_s):
     npair = mol.n_shell * (mol.n_shell + 1) // 2
     stride = max(1, npair // 64)
     t = time.perf_counter()
-    o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
+    o.make_r_JK_sample(P, stride, schwarz=schwarz, max_I=max_I)
     full_est = (time.perf_counter() - t) * stride
     stride = max(1, int(math.ceil(full_est / target_s)))
     # the cost per bra pair varies widely: refine the stride until the sample takes at
     # least half the target
     for _ in range(3):
         t = time.perf_counter()
-        st = o.make_r_JK_sample(P, stride, schwarz=SCHWARZ, max_I=max_I)
+        st = o.make_r_JK_sample(P, stride, schwarz=schwarz, max_I=max_I)
         dt = time.perf_counter() - t
         if dt >= 0.5 * target_s or stride == 1:
             break
@@ -80,7 +80,7 @@ def cpu_baseline(mol, P, max_I, target_s):
     O.set_threads(1)
     try:
         t = time.perf_counter()
-        st1 = o.make_r_JK_sample(P, 20 * stride, schwarz=SCHWARZ, max_I=max_I)
+        st1 = o.make_r_JK_sample(P, 20 * stride, schwarz=schwarz, max_I=max_I)
         dt1 = time.perf_counter() - t
     finally:
         O.set_threads(cores)
@@ -219,6 +219,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--waters", type=int, default=N_WATER, help="cluster size (default 64)")
+    ap.add_argument("--basis", default=BASIS, help="basis of the cluster (default cc-pVTZ; other "
+                    "workloads are for profiling, the headline line is the default)")
+    ap.add_argument("--schwarz", type=float, default=SCHWARZ, help="Schwarz cutoff (default 1e-11)")
     ap.add_argument("--density", default=None,
                     help="converged density (.npy) to time the steps at; skips both SCF runs")
     ap.add_argument("--save-density", default=None)
@@ -226,7 +229,7 @@ def main():
                     help="profiling runs: skip both timed SCFs, time the steps at the density "
                          "of this many FP64 SCF iterations from the promolecule guess")
     ap.add_argument("--no-side", action="store_true", help="skip the side legs (configs 2-4)")
-    ap.add_argument("--guess-clusters", type=int, default=10,
+    ap.add_argument("--guess-clusters", type=int, default=50,
                     help="held-out clusters of the learned-guess leg (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM traffic "
@@ -255,9 +258,9 @@ def main():
         print(json.dumps({"copy_mib": a.copy_calibration_mib, "dispatches": 3,
                           "bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8}))
         return
-    mol = water_cluster(a.waters, BASIS, seed=SEED)
-    log(f"workload: (H2O){a.waters} {BASIS} n_bf={mol.n_bf} n_shell={mol.n_shell}")
-    scf = GPUSCF(mol, schwarz=SCHWARZ, fp32_threshold=FP32_THR)
+    mol = water_cluster(a.waters, a.basis, seed=SEED)
+    log(f"workload: (H2O){a.waters} {a.basis} n_bf={mol.n_bf} n_shell={mol.n_shell}")
+    scf = GPUSCF(mol, schwarz=a.schwarz, fp32_threshold=FP32_THR)
     log(f"setup (1e ints + Schwarz) {scf.setup_s:.2f}s")
     extra = {}
     eng = scf.engine
@@ -361,10 +364,10 @@ def main():
     if tr is not None and mol.n_bf == 4160:
         roof["traffic"] = tr["bytes"]
         roof["traffic_read_written"] = [tr["read"], tr["written"]]
-    cfg = {"workload": f"(H2O){a.waters} RHF/{BASIS} Cartesian (f on O), {mol.n_bf} basis functions, "
-                       f"seeded cubic-lattice cluster, Schwarz {SCHWARZ:g}; one step = one J/K build at the "
+    cfg = {"workload": f"(H2O){a.waters} RHF/{a.basis} Cartesian, {mol.n_bf} basis functions, "
+                       f"seeded cubic-lattice cluster, Schwarz {a.schwarz:g}; one step = one J/K build at the "
                        f"converged density, FP32 below Schwarz bound {FP32_THR:g}",
-           "n_bf": mol.n_bf, "schwarz_cutoff": SCHWARZ,
+           "n_bf": mol.n_bf, "schwarz_cutoff": a.schwarz,
            "quartets_per_step": q, "prim_quartets_per_step": stmx["prim_quartets"],
            "fp32_quartets_per_step": stmx["fp32_quartets"],
            "build_tflops_mixed": fmx / (ms_step * 1e-3) / 1e12, **extra}
@@ -385,7 +388,7 @@ def main():
         if a.guess_clusters > 0:
             out["config"]["learned_guess"] = guess_leg(a.guess_clusters, log)
     if not a.no_cpu:
-        cb = cpu_baseline(mol, P.cpu().numpy(), scf.max_I, a.cpu_seconds)
+        cb = cpu_baseline(mol, P.cpu().numpy(), scf.max_I, a.cpu_seconds, a.schwarz)
         log(f"cpu baseline {cb['value']:.3e} quartets/s on {cb['cores']} threads, "
             f"{cb['one_thread']['value']:.3e} on one ({cb['seconds']:.1f}s)")
         cb.pop("seconds")

@@ -37,7 +37,8 @@ def _log(*a):
 
 def _generate_part(args):
     """Training rows of the clusters `idx` (indices into the seeded size list), in a
-    worker process of its own: (rows, SCF iterations, skipped indices)."""
+    worker process of its own: (rows, SCF iterations, skipped indices); prints its
+    progress every 25 clusters."""
     idx, sizes, seed0 = args
     import torch
     from .driver import GPUSCF
@@ -45,7 +46,10 @@ def _generate_part(args):
     rows = {t: ([], []) for t in G.TYPES}
     its, skipped = [], []
     atom = None
-    for i in idx:
+    t0 = time.perf_counter()
+    for k, i in enumerate(idx):
+        if k and k % 25 == 0:
+            print(f"data: worker at cluster {k}/{len(idx)}, {time.perf_counter() - t0:.0f}s", flush=True)
         mol = water_cluster(int(sizes[i]), BASIS, seed=seed0 + int(i))
         s = GPUSCF(mol)
         if atom is None:
