@@ -82,6 +82,29 @@ def promolecule_guess(mol, atom_scf):
     return torch.block_diag(*blocks)
 
 
+def _gram_append(gram, errs):
+    """Gram matrix of the error vectors errs (tensors, or tuples of tensors whose inner
+    products add up) after the last one was appended: only the new row is computed, on
+    the device (no host round trip)."""
+    def dot(a, b):
+        if isinstance(a, tuple):
+            return sum(torch.sum(x * y) for x, y in zip(a, b))
+        return torch.sum(a * b)
+    new = torch.stack([dot(e, errs[-1]) for e in errs])
+    if gram is None:
+        return new.reshape(1, 1)
+    return torch.cat([torch.cat([gram, new[:-1, None]], 1), new[None, :]], 0)
+
+
+def _diis_matrix(gram):
+    """Pulay's bordered matrix [[B, -1], [-1, 0]] from the Gram matrix B."""
+    m = gram.shape[0]
+    B = torch.full((m + 1, m + 1), -1.0, dtype=gram.dtype, device=gram.device)
+    B[:m, :m] = gram
+    B[m, m] = 0.0
+    return B
+
+
 def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e-6,
             max_iterations=100, min_iterations=1, diis_keep=8, diis_start=3,
             diis_start_error=None,
@@ -146,6 +169,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
     E = 0.5 * torch.sum(P * (F + h)).item() + e_nuc
     history = []
     diis_F, diis_E = [], []
+    gram = None  # Gram matrix of the kept DIIS error vectors, on the device
     P_build = P
     since_full = 0  # builds since the last full one
     loose_G = low and loose_scale != 1.0  # G holds a loose-cutoff build
@@ -161,21 +185,17 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
         err_max = err.abs().max().item()
         diis_F.append(F)
         diis_E.append(err)
+        gram = _gram_append(gram, diis_E)
         if len(diis_F) > diis_keep:
             diis_F.pop(0)
             diis_E.pop(0)
+            gram = gram[1:, 1:]
         F_use = F
         if diis_start_error is not None and err_max < diis_start_error:
             diis_on = True
         if it >= diis_start and len(diis_F) >= 2 and diis_on:
             m = len(diis_F)
-            B = torch.empty((m + 1, m + 1), dtype=dt, device=h.device)
-            for i in range(m):
-                for j in range(i + 1):
-                    B[i, j] = B[j, i] = torch.sum(diis_E[i] * diis_E[j])
-            B[m, :] = -1.0
-            B[:, m] = -1.0
-            B[m, m] = 0.0
+            B = _diis_matrix(gram)
             rhs = torch.zeros(m + 1, dtype=dt, device=h.device)
             rhs[m] = -1.0
             try:
@@ -232,6 +252,7 @@ def run_rhf(h, S, n_electrons, e_nuc, jk, P0=None, convergence=1e-8, diis_tol=1e
                 if diis_reset_on_switch:
                     diis_F.clear()
                     diis_E.clear()
+                    gram = None
             done = False
         elif done and not last_full_fp64 and final_full:
             # win back the incremental error: a full FP64 build on the current density
@@ -290,6 +311,7 @@ def run_uhf(h, S, n_alpha, n_beta, e_nuc, jk, Pa0=None, Pb0=None, convergence=1e
     t0 = time.perf_counter()
     Fa, Fb, E = fock(Pa, Pb)
     hist_F, hist_E = [], []
+    gram = None  # Gram matrix of the kept (alpha, beta) error pairs
     history = []
     converged = False
     it = 0
@@ -300,20 +322,15 @@ def run_uhf(h, S, n_alpha, n_beta, e_nuc, jk, Pa0=None, Pb0=None, convergence=1e
         err = max(ea.abs().max().item(), eb.abs().max().item())
         hist_F.append((Fa, Fb))
         hist_E.append((ea, eb))
+        gram = _gram_append(gram, hist_E)
         if len(hist_F) > diis_keep:
             hist_F.pop(0)
             hist_E.pop(0)
+            gram = gram[1:, 1:]
         Fa_u, Fb_u = Fa, Fb
         if it >= diis_start and len(hist_F) >= 2:
             m = len(hist_F)
-            B = torch.empty((m + 1, m + 1), dtype=dt, device=h.device)
-            for i in range(m):
-                for j in range(i + 1):
-                    B[i, j] = B[j, i] = (torch.sum(hist_E[i][0] * hist_E[j][0]) +
-                                         torch.sum(hist_E[i][1] * hist_E[j][1]))
-            B[m, :] = -1.0
-            B[:, m] = -1.0
-            B[m, m] = 0.0
+            B = _diis_matrix(gram)
             rhs = torch.zeros(m + 1, dtype=dt, device=h.device)
             rhs[m] = -1.0
             try:
