@@ -185,13 +185,18 @@ int tgpu_get_class_counters(tgpu_ctx *ctx, unsigned long long *out, int n);
 int tgpu_set_profiling(tgpu_ctx *ctx, int on);
 int tgpu_get_class_times(tgpu_ctx *ctx, double *ms, int n);
 
-/* Per-class-pair kernel form: 0 = one lane per quartet (classes with <= 81 integrals),
- * 1 = one wavefront per quartet (every class).  tgpu_autotune times both forms on a build
- * of the given device density and keeps the faster one per class pair; ms_out (optional,
- * 2 doubles per class pair) receives the two times (-1 where form 0 does not exist).
+/* Per-class-pair kernel form: 0 = one lane per quartet (class pairs with <= 200 integrals
+ * and (e0|f0) entries), 1 = one wavefront per quartet (every class pair); and, for the
+ * wavefront form, the ket chunk a work-group takes at a time (16, 32, 64; 0 = chosen per
+ * launch from the grid size).  tgpu_autotune times both forms on a build of the given
+ * device density and keeps the faster one per class pair, then times the ket chunks of
+ * the wavefront-form class pairs and keeps the fastest; ms_out (optional, 2 doubles per
+ * class pair) receives the two forms' times (-1 where form 0 does not exist).
  * No Tonto counterpart: kernel selection. */
 int tgpu_set_variant(tgpu_ctx *ctx, int class_pair, int variant);
 int tgpu_get_variant(const tgpu_ctx *ctx, int class_pair);
+int tgpu_set_ket_chunk(tgpu_ctx *ctx, int class_pair, int chunk);
+int tgpu_get_ket_chunk(const tgpu_ctx *ctx, int class_pair);
 int tgpu_autotune(tgpu_ctx *ctx, const double *P_device, int flags, double *ms_out);
 
 /* One-electron matrices S, T, V (n_bf*n_bf doubles, host arrays), the core-Hamiltonian

@@ -361,15 +361,21 @@ def test_autotune_keeps_results(torch_cuda):
     J0, K0 = e.make_r_JK(g["P"])
     P = torch.tensor(g["P"], device="cuda")
     ms, var = e.autotune(P.data_ptr())
-    print("autotune variants", var)
+    chunks = e.ket_chunks()
+    print("autotune variants", var, "ket chunks", chunks)
     assert all(v in (0, 1) for v in var)
+    assert all(c in (0, 16, 32, 64) for c in chunks)
+    assert all(c == 0 for v, c in zip(var, chunks) if v == 0)
     J1, K1 = e.make_r_JK(g["P"])
     assert np.max(np.abs(J1 - J0)) < 1e-12 and np.max(np.abs(K1 - K0)) < 1e-12
-    # every light class pair also through the wavefront form
+    # every light class pair also through the wavefront form, with every ket chunk
     for k in range(len(var)):
         e.set_variant(k, 1)
-    J2, K2 = e.make_r_JK(g["P"])
-    assert np.max(np.abs(J2 - J0)) < 1e-12 and np.max(np.abs(K2 - K0)) < 1e-12
+    for chunk in (16, 32, 64, 0):
+        for k in range(len(var)):
+            e.set_ket_chunk(k, chunk)
+        J2, K2 = e.make_r_JK(g["P"])
+        assert np.max(np.abs(J2 - J0)) < 1e-12 and np.max(np.abs(K2 - K0)) < 1e-12, chunk
 
 
 @pytest.mark.parametrize("only", ["J", "K"])

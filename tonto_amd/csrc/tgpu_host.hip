@@ -443,6 +443,8 @@ struct tgpu_ctx {
     bool have_schwarz = false;
     std::vector<double> maxI_mol, maxI_tonto;
     int variant[NQCLS];
+    // wavefront form: kets per work-group chunk per class pair (0: chosen per launch)
+    int chunk[NQCLS] = {0};
     // device
     DBuf<int> d_rec0, d_rec1, d_sh_l, d_sh_bf, d_seg_start, d_seg_count, d_nbf, d_shell_off,
         d_atom_off, d_kcol, d_kcol_map;
@@ -1198,12 +1200,16 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
                 // the LDS-row form is for single large molecules (0 keeps batches of many,
                 // mostly small molecules at four bra pairs per work-group)
                 a.n_ket_avg = c->n_mol == 1 ? c->bin_start[Y + 1] - c->bin_start[Y] : 0;
-                // wavefront form: one lane screens each ket of a chunk; chunks of 64 unless
-                // that leaves fewer than ~128k work-groups, then 32 or 16
+                // wavefront form: one lane screens each ket of a chunk; the class pair's
+                // tuned chunk (tgpu_autotune), else chunks of 64 unless that leaves fewer
+                // than ~128k work-groups, then 32 or 16
                 a.ket_chunk = 64;
-                while (a.ket_chunk > 16 &&
-                       (long long)a.n_bra * ((a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk) < 131072)
-                    a.ket_chunk /= 2;
+                if (c->chunk[k] > 0)
+                    a.ket_chunk = c->chunk[k];
+                else
+                    while (a.ket_chunk > 16 &&
+                           (long long)a.n_bra * ((a.n_ket_max + a.ket_chunk - 1) / a.ket_chunk) < 131072)
+                        a.ket_chunk /= 2;
                 if (a.n_bra == 0 || c->bin_start[Y + 1] == c->bin_start[Y]) continue;
                 if (kLaunch[k](a, c->variant[k], mixed, s)) {
                     c->err = "class pair " + std::to_string(k) + ": no wavefront-form workspace";
@@ -1513,8 +1519,24 @@ extern "C" int tgpu_get_variant(const tgpu_ctx *c, int k)
     return c->variant[k];
 }
 
+extern "C" int tgpu_set_ket_chunk(tgpu_ctx *c, int k, int chunk)
+{
+    if (!c || k < 0 || k >= NQCLS || !(chunk == 0 || chunk == 16 || chunk == 32 || chunk == 64)) return TGPU_E_ARG;
+    c->chunk[k] = chunk;
+    drop_graph(c);
+    return TGPU_OK;
+}
+
+extern "C" int tgpu_get_ket_chunk(const tgpu_ctx *c, int k)
+{
+    if (!c || k < 0 || k >= NQCLS) return -1;
+    return c->chunk[k];
+}
+
 // Autotuner: for every class pair that has both forms, time one build with each form on
-// the given density (device pointer) and keep the faster one.  ms_out (optional,
+// the given density (device pointer) and keep the faster one; then, for the class pairs
+// in the wavefront form, time the ket chunk (the kets one work-group takes at a time: 16,
+// 32, 64, or the per-launch rule) and keep the fastest.  ms_out (optional,
 // 2 * n_class_pairs doubles) receives the measured times of variant 0 and 1.
 extern "C" int tgpu_autotune(tgpu_ctx *c, const double *P, int flags, double *ms_out)
 {
@@ -1544,6 +1566,24 @@ extern "C" int tgpu_autotune(tgpu_ctx *c, const double *P, int flags, double *ms
             ms_out[2 * k] = kHeavy[k] ? -1.0 : t[0][k];
             ms_out[2 * k + 1] = t[1][k];
         }
+    }
+    // ket chunk of the wavefront-form class pairs (candidate 0 = the per-launch rule)
+    static const int kChunks[4] = {0, 16, 32, 64};
+    double tc[4][NQCLS];
+    for (int q = 0; q < 4; ++q) {
+        for (int k = 0; k < NQCLS; ++k) c->chunk[k] = kChunks[q];
+        if ((rc = enqueue_build(c, ctx_bufs(c), P, c->d_hJ.p, c->d_hK.p, flags & TGPU_JK_MIXED, c->stream,
+                                true, true)))
+            return rc;
+        c->timed = true;
+        HIPCHK(c, hipGetLastError());
+        if ((rc = tgpu_get_class_times(c, tc[q], NQCLS))) return rc;
+    }
+    for (int k = 0; k < NQCLS; ++k) {
+        int best = 0;
+        for (int q = 1; q < 4; ++q)  // a fixed chunk must beat the rule by 3% (timing noise)
+            if (tc[q][k] < (best == 0 ? 0.97 : 1.0) * tc[best][k]) best = q;
+        c->chunk[k] = c->variant[k] == 1 ? kChunks[best] : 0;
     }
     c->profiling = was;
     drop_graph(c);

@@ -24,7 +24,7 @@ EXPORTS = ["tgpu_create", "tgpu_destroy", "tgpu_last_error", "tgpu_dims", "tgpu_
            "tgpu_one_electron",
            "tgpu_get_class_counters", "tgpu_n_class_pairs", "tgpu_set_profiling",
            "tgpu_get_class_times", "tgpu_create_batch", "tgpu_batch_layout", "tgpu_set_variant",
-           "tgpu_get_variant", "tgpu_autotune"]
+           "tgpu_get_variant", "tgpu_set_ket_chunk", "tgpu_get_ket_chunk", "tgpu_autotune"]
 
 _lib = None
 
@@ -83,6 +83,8 @@ def lib():
                                         ctypes.POINTER(ctypes.c_longlong), vp, vp, vp]
         L.tgpu_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tgpu_get_variant.argtypes = [vp, ctypes.c_int]
+        L.tgpu_set_ket_chunk.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.tgpu_get_ket_chunk.argtypes = [vp, ctypes.c_int]
         L.tgpu_autotune.argtypes = [vp, vp, ctypes.c_int, vp]
         _lib = L
     return _lib
@@ -237,6 +239,13 @@ class JKEngine:
 
     def set_variant(self, k, v):
         self._check(lib().tgpu_set_variant(self._ctx, k, v))
+
+    def ket_chunks(self):
+        """Per class pair: the wavefront form's ket chunk (0: chosen per launch)."""
+        return [lib().tgpu_get_ket_chunk(self._ctx, k) for k in range(lib().tgpu_n_class_pairs())]
+
+    def set_ket_chunk(self, k, chunk):
+        self._check(lib().tgpu_set_ket_chunk(self._ctx, k, chunk))
 
     def stats(self):
         s = Stats()
