@@ -355,8 +355,26 @@ struct JKArgs {
     const double *P, *Pmax, *scal;
     double cutoff, jp, kp, fp32_thr;
     double *X, *Y;
+    // accumulator copies (single molecules): X and Y hold TGPU_ACC_COPIES copies acc_stride
+    // doubles apart and a wavefront adds into the copy of the XCD it runs on (0: one copy)
+    long long acc_stride;
     unsigned long long *counters;
 };
+
+// Copies of the J/K accumulators: one per XCD (accelerator complex die).  Device-scope
+// atomics from different XCDs on one cache line move the line between the XCDs' L2
+// caches, one memory write per update; with a copy per XCD each copy's lines are updated
+// by one XCD only.  Any copy index gives the same sums (the atomics stay device-scope),
+// so the index only has to be in range.
+#define TGPU_ACC_COPIES 8
+
+// The XCD the wavefront runs on (hardware register XCC_ID), reduced to a copy index.
+__device__ __forceinline__ int acc_copy()
+{
+    unsigned r;
+    asm("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(r));
+    return (int)(r & (TGPU_ACC_COPIES - 1));
+}
 
 // Everything a wavefront needs about its bra pair, and the Schwarz frontier of its kets.
 struct BraInfo {
@@ -377,8 +395,9 @@ __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
     const long long mo = ldc(a.ml.mat_off + m);
     b.ld = ldc(a.ml.nbf + m);
     b.P = a.P + mo;
-    b.X = a.X + mo;
-    b.Y = a.Y + mo;
+    const long long co = a.acc_stride ? (long long)acc_copy() * a.acc_stride : 0;
+    b.X = a.X + co + mo;
+    b.Y = a.Y + co + mo;
     b.Pmax = a.Pmax + ldc(a.ml.pair_off + m);
     const double P_max = ldc(a.scal + 2 * m);
     b.IP_max = ldc(a.scal + 2 * m + 1);

@@ -160,6 +160,21 @@ __global__ void post_kernel(const double *__restrict__ X, const double *__restri
     }
 }
 
+// Accumulator copies (X and Y each hold `copies` copies of n elements): the sum into the
+// first copy, before post_kernel.
+__global__ void acc_reduce_kernel(double *__restrict__ X, double *__restrict__ Y, size_t n, int copies)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double x = X[i], y = Y[i];
+        for (int c = 1; c < copies; ++c) {
+            x += X[c * n + i];
+            y += Y[c * n + i];
+        }
+        X[i] = x;
+        Y[i] = y;
+    }
+}
+
 // Spherical basis (one molecule): Cartesian <-> real solid harmonic transforms of a
 // symmetric matrix.  T is block diagonal (one (ncart x nsph) block per shell, the unit
 // matrix for s and p): P_cart = T P_sph T^T before a build, J_sph = T^T J_cart T after
@@ -452,6 +467,7 @@ struct tgpu_ctx {
     DBuf<double> d_A, d_AB, d_maxI, d_maxI_mol;
     DBuf<PrimRec> d_prim;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
+    int acc_copies = 1;  // accumulator copies in d_X, d_Y (1 or TGPU_ACC_COPIES)
     DBuf<double> d_hP2, d_hJ2, d_hK2, d_uJ, d_uK;  // UHF: host-entry staging, ket-side pass
     DBuf<unsigned long long> d_counters;
     // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
@@ -866,7 +882,15 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
         (rc = dupload(c, c->d_atom_off, c->atom_off)) || (rc = dupload(c, c->d_pair_off, c->pair_off)) ||
         (rc = dupload(c, c->d_mat_off, c->mat_off)))
         return rc;
-    if ((rc = dalloc(c, c->d_X, (size_t)c->n_elem)) || (rc = dalloc(c, c->d_Y, (size_t)c->n_elem)) ||
+    // single molecules: one accumulator copy per XCD while the copies stay under 4 GiB
+    // (TGPU_ACC_COPIES=1 in the environment keeps one)
+    c->acc_copies = n_mol == 1 && 2 * TGPU_ACC_COPIES * (size_t)c->n_elem * sizeof(double) <= (4ull << 30)
+                        ? TGPU_ACC_COPIES
+                        : 1;
+    if (const char *ac = std::getenv("TGPU_ACC_COPIES"))
+        if (std::atoi(ac) == 1) c->acc_copies = 1;
+    if ((rc = dalloc(c, c->d_X, (size_t)c->n_elem * c->acc_copies)) ||
+        (rc = dalloc(c, c->d_Y, (size_t)c->n_elem * c->acc_copies)) ||
         (rc = dalloc(c, c->d_Pmax, (size_t)c->n_pairs)) || (rc = dalloc(c, c->d_scal, 2 * (size_t)n_mol)) ||
         (rc = dalloc(c, c->d_maxI_mol, (size_t)n_mol)) || (rc = dalloc(c, c->d_counters, 3 * NQCLS)))
         return rc;
@@ -1098,11 +1122,13 @@ struct BuildBufs {
     unsigned long long *counters;
     double *ws;
     long long ws_doubles;
+    int acc_copies;  // copies of X and Y (n_elem doubles apart)
 };
 
 static BuildBufs ctx_bufs(const tgpu_ctx *c)
 {
-    return BuildBufs{c->d_X.p, c->d_Y.p, c->d_Pmax.p, c->d_scal.p, c->d_counters.p, c->d_ws.p, c->ws_doubles};
+    return BuildBufs{c->d_X.p,       c->d_Y.p,    c->d_Pmax.p,      c->d_scal.p,
+                     c->d_counters.p, c->d_ws.p, c->ws_doubles, c->acc_copies};
 }
 
 // Largest per-slot workspace (doubles) of the wavefront-form class pairs this basis has
@@ -1142,8 +1168,9 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
         ba.sh_sbf = c->d_sh_sbf.p;
         ba.n_sph = c->n_sph;
     }
-    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.X, c->n_elem);
-    fill_zero_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.Y, c->n_elem);
+    const long long n_acc = c->n_elem * B.acc_copies;
+    fill_zero_kernel<<<grid_for(n_acc), 256, 0, s>>>(B.X, n_acc);
+    fill_zero_kernel<<<grid_for(n_acc), 256, 0, s>>>(B.Y, n_acc);
     fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(B.counters, 3 * NQCLS);
     pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(c->spherical ? Psph : P, P2, ba, B.Pmax);
     mol_max_kernel<<<c->n_mol, 256, 0, s>>>(B.Pmax, ba, c->d_maxI_mol.p, B.scal);
@@ -1168,6 +1195,7 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     a.fp32_thr = c->fp32_thr;
     a.X = B.X;
     a.Y = B.Y;
+    a.acc_stride = B.acc_copies > 1 ? c->n_elem : 0;
     a.counters = B.counters;
     a.ws = B.ws;
     a.ws_doubles = B.ws_doubles;
@@ -1222,6 +1250,8 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
         }
     }
     if (rec_end) (void)hipEventRecord(c->ev1, s);
+    if (B.acc_copies > 1)
+        acc_reduce_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.X, B.Y, (size_t)c->n_elem, B.acc_copies);
     post_kernel<<<dim3(grid_for(c->n_elem / c->n_mol + 1), c->n_mol), 256, 0, s>>>(B.X, B.Y, ba, J, K);
     if (c->spherical) {
         const SphMaps sm = sph_maps(c);
@@ -1352,6 +1382,7 @@ extern "C" int tgpu_make_r_JK_device_noalloc(tgpu_ctx *c, const double *P, doubl
     B.counters = reinterpret_cast<unsigned long long *>(base + off); off += b_cnt;
     B.ws = ws_slot ? reinterpret_cast<double *>(base + off) : nullptr;
     B.ws_doubles = ws_slot ? (long long)((TGPU_NOALLOC_SCRATCH_BYTES - off) / sizeof(double)) : 0;
+    B.acc_copies = 1;
     const int kflags = flags & (TGPU_JK_MIXED | TGPU_J_ONLY | TGPU_K_ONLY);
     if (int rc = enqueue_build(c, B, P, J, K, kflags, c->stream, true, true)) return rc;
     // the context's own counters take this build's, so tgpu_get_stats reports it
