@@ -515,18 +515,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     __shared__ int2 kqueue[4][128];
 #endif
     extern __shared__ double krow[];  // KROWS: [NP + NQ][ld]
-    for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
-        rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
     const int lane = threadIdx.x & 63;
     // the bra pair is uniform over the wavefront: readfirstlane lets the compiler keep
     // everything derived from it in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = KROWS ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave;
-    if (KROWS && i >= a.n_bra) return;  // uniform over the work-group
-    __syncthreads();
-    if (i >= a.n_bra) return;
     BraInfo b;
-    load_bra(a, i, b);
+    if (KROWS) {
+        if (i >= a.n_bra) return;  // uniform over the work-group
+        load_bra(a, i, b);
+        // a bra pair with no kets left by the Schwarz prefix leaves before staging the
+        // Rys table and zeroing its K rows (2.5 % of an (H2O)64/cc-pVTZ build)
+        if (b.jend == 0) return;  // uniform
+    }
+    for (int t = threadIdx.x; t < rys_block_size(NRT); t += blockDim.x)
+        rys_lds[t] = rys_table_data[rys_table_offset[NRT] + t];
+    __syncthreads();
+    if (!KROWS) {
+        if (i >= a.n_bra) return;
+        load_bra(a, i, b);
+    }
     const int ld = b.ld, fp = b.fp, fq = b.fq;
     const int nkr = KROWS ? a.krow_rows : 0;  // K rows accumulated in LDS
     // LDS row: the nR functions of momentum LR, then (LS != LR) those of momentum LS
