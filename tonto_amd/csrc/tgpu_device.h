@@ -414,6 +414,9 @@ __device__ __forceinline__ void load_bra(const JKArgs &a, int i, BraInfo &b)
     // kets are sorted by max_I descending, so those passing
     // max_I(ab)*max_I(cd)*P_max >= cutoff (molecule.fock.foo:1430) form a prefix
     int lo = 0, hi = jlim;
+    // most bra pairs far down the sorted bin pass no ket at all: one test of the largest
+    // ket settles that without the search
+    if (hi > 0 && b.mIb * ldc(b.maxI) * P_max < a.cutoff) hi = 0;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (b.mIb * ldc(b.maxI + mid) * P_max < a.cutoff) hi = mid;
