@@ -47,7 +47,7 @@ SCHWARZ, FP32_THR = 1e-11, 1e-6
 # launches of one build, 16 dispatches), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes
 # scaled by what the same counters read for a 256 MiB copy (x2.000 and x1.000): the
 # profiles/ traffic summary of (H2O)64 / cc-pVTZ
-TRAFFIC = {"jk class (10|00) mixed": {"bytes": 1.677406e+12, "read": 1.610175e+12, "written": 6.723150e+10,
+TRAFFIC = {"jk class (10|00) mixed": {"bytes": 1.680240e+12, "read": 1.613008e+12, "written": 6.723149e+10,
                                      "dispatches": 16}}
 
 
