@@ -58,7 +58,7 @@ def test_water_all_classes(torch_cuda, oracle_mod, basis):
     _check_jk(oracle_mod, mol, _rand_density(mol.n_bf, 1))
 
 
-@pytest.mark.parametrize("budget", [None, 16690, 26450])
+@pytest.mark.parametrize("budget", [None, 16690, 24032, 26450])
 def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
     """The lane-per-quartet kernels' LDS K-row form on (H2O)_8 / 6-31G(d) (152 functions;
     56 s-type, 48 p-type, 48 d-type), made to run for every ket segment length.  A row
@@ -67,7 +67,8 @@ def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
     4 rows of 56: (pp|ss) keeps 4 of 6 rows, (ds|ss) 4 of 7, (ps|ps) (104 columns) 2 of 4 in
     LDS, the rest go to global atomics; 26450 B, next to the 22240 B of the three-root
     classes: (dp|ps) (104 columns) keeps 5 of 9 rows, (dd|ss) 9 of 12, (fp|ss) 9 of 13,
-    (fd|ss) 9 of 16."""
+    (fd|ss) 9 of 16; 24032 B: 4 rows of 56 next to the three-root classes, fewer than half
+    of them: (dd|ss) 4 of 12, (fp|ss) 4 of 13, (fd|ss) 4 of 16 (the form needs a quarter)."""
     import os
     from tonto_amd.geometry import water_cluster
     g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))

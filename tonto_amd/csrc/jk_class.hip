@@ -25,8 +25,9 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
         // The LDS-row form for single molecules with long ket segments (>= 4 rounds of 64 by
         // default, so that the four wavefronts of one bra pair have work): as many of the
         // bra pair's K rows as fit the work-group's LDS budget next to the Rys table and
-        // the ket queue, and at least half of them.  A row holds the columns of the ket
-        // shells' two angular momenta only.  Not built for (ff|ss): its FP64 instantiation
+        // the ket queue, and at least a quarter of them ((fd|ps) at 4160 functions: 7 of
+        // 16 rows in LDS, 120 -> 71 ms against every K update global).  A row holds the
+        // columns of the ket shells' two angular momenta only.  Not built for (ff|ss): its FP64 instantiation
         // gives wrong K on every cluster tested while its mixed twin and the global-atomic
         // form are exact (cause not found), so that class pair updates K globally.
 #if !(TGPU_LP == 3 && TGPU_LQ == 3 && TGPU_LR == 0 && TGPU_LS == 0)
@@ -50,7 +51,7 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
         const int fit = a.kcol_map && W > 0 && budget > kStatic ? (int)((budget - kStatic) / row_bytes) : 0;
         const int nkr = fit < NP + NQ ? fit : NP + NQ;
         // (no K digested, as in the UHF ket-side J pass: no rows)
-        if (a.kp != HUGE_VAL && a.n_ket_avg >= a.krow_min_kets && nkr > 0 && 2 * nkr >= NP + NQ) {
+        if (a.kp != HUGE_VAL && a.n_ket_avg >= a.krow_min_kets && nkr > 0 && 4 * nkr >= NP + NQ) {
             JKArgs b = a;
             b.krow_rows = nkr;
             const size_t rows = (size_t)nkr * row_bytes;
