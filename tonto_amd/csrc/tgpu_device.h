@@ -843,6 +843,17 @@ __host__ __device__ constexpr long long ws_slot_count(long long ws_doubles, long
 #define TGPU_WAVE_TEAM_LDS (16 * 1024)
 #endif
 
+// Class pairs whose wavefront form runs faster with up to 8 (e0|f0) entries per lane in
+// registers than with 4 (FP64 build at (H2O)64/cc-pVTZ: (dp|dp) 264 -> 250 ms, (fs|dp)
+// 208 -> 193, (dd|ds) 199 -> 184, (fp|pp) 135 -> 124, (fp|fs) 68 -> 63, (fd|dp) 175 ->
+// 170; five other class pairs were 5-24 ms slower with 8)
+__host__ __device__ constexpr bool wave_greg8(int lp, int lq, int lr, int ls)
+{
+    return (lp == 2 && lq == 1 && lr == 2 && ls == 1) || (lp == 3 && lq == 0 && lr == 2 && ls == 1) ||
+           (lp == 2 && lq == 2 && lr == 2 && ls == 0) || (lp == 3 && lq == 1 && lr == 1 && ls == 1) ||
+           (lp == 3 && lq == 1 && lr == 3 && ls == 0) || (lp == 3 && lq == 2 && lr == 2 && ls == 1);
+}
+
 template <int LP, int LQ, int LR, int LS>
 struct WaveCfg {
     using T = WaveTab<LP, LQ, LR, LS>;
@@ -866,9 +877,10 @@ struct WaveCfg {
                                  : (2 * TEAM_BYTES <= TGPU_WAVE_TEAM_LDS && 3 * T::NROOT <= 32) ? 2
                                                                                                 : 1;
     static constexpr int S = 64 / TEAMS;
-    // (e0|f0) entries per lane; in registers up to TGPU_WAVE_GREG_MAX, else in the buffer
+    // (e0|f0) entries per lane; in registers up to TGPU_WAVE_GREG_MAX (8 for the class
+    // pairs of wave_greg8), else in the buffer
     static constexpr int MG = (NG + S - 1) / S;
-    static constexpr bool GREG = MG <= TGPU_WAVE_GREG_MAX;
+    static constexpr bool GREG = MG <= (wave_greg8(LP, LQ, LR, LS) && TGPU_WAVE_GREG_MAX < 8 ? 8 : TGPU_WAVE_GREG_MAX);
 };
 
 // One team's 2-D tables and transfer buffers, in the arithmetic type R of its quartet
