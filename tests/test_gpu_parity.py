@@ -68,7 +68,9 @@ def test_lds_row_form_full_and_partial_rows(torch_cuda, oracle_mod, budget):
     LDS, the rest go to global atomics; 26450 B, next to the 22240 B of the three-root
     classes: (dp|ps) (104 columns) keeps 5 of 9 rows, (dd|ss) 9 of 12, (fp|ss) 9 of 13,
     (fd|ss) 9 of 16; 24032 B: 4 rows of 56 next to the three-root classes, fewer than half
-    of them: (dd|ss) 4 of 12, (fp|ss) 4 of 13, (fd|ss) 4 of 16 (the form needs a quarter)."""
+    of them: (dd|ss) 4 of 12, (fp|ss) 4 of 13, (fd|ss) 4 of 16 (the form needs a quarter).
+    Uncapped, the (..|ss) class pairs also read their density rows from LDS (every bra
+    pair: the segment floor is 0)."""
     import os
     from tonto_amd.geometry import water_cluster
     g = np.load(os.path.join(HERE, "golden", "w8_631gd.npz"))

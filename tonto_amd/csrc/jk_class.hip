@@ -54,7 +54,13 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
         if (a.kp != HUGE_VAL && a.n_ket_avg >= a.krow_min_kets && nkr > 0 && 4 * nkr >= NP + NQ) {
             JKArgs b = a;
             b.krow_rows = nkr;
-            const size_t rows = (size_t)nkr * row_bytes;
+            // (..|ss) class pairs: density rows in the LDS left after the K rows, when all
+            // of both fit
+            const int pfit = (int)((budget - kStatic - (size_t)nkr * row_bytes) / row_bytes);
+            b.prow_rows = TGPU_LR == 0 && TGPU_LS == 0 && a.prow_enable && nkr == NP + NQ && pfit >= NP + NQ
+                              ? NP + NQ
+                              : 0;
+            const size_t rows = (size_t)(nkr + b.prow_rows) * row_bytes;
             if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(b);
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(b);
             return 0;

@@ -336,6 +336,9 @@ struct JKArgs {
     int krow_rows;   // KROWS form: the first krow_rows K rows of the bra pair live in LDS
     int krow_budget; // cap on a work-group's LDS in the KROWS form (bytes; 0: default)
     int krow_min_kets;  // shortest ket segment (n_ket_avg) for the KROWS form
+    // KROWS form: density rows staged in LDS next to the K rows (set per launch;
+    // prow_enable 0 keeps every density read global)
+    int prow_rows, prow_enable;
     // KROWS form, single molecules: the K columns a class pair touches are the functions
     // of its ket shells' two angular momenta, so an LDS row holds only those, numbered
     // within each momentum.  kcol[pair] = pack16(number of the pair's first function
@@ -543,8 +546,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     // LDS row: the nR functions of momentum LR, then (LS != LR) those of momentum LS
     const int nR = KROWS ? a.kcol_lofs[LR + 1] - a.kcol_lofs[LR] : 0;
     const int W = KROWS ? (LR == LS ? nR : nR + a.kcol_lofs[LS + 1] - a.kcol_lofs[LS]) : 0;
+    // P rows (KROWS with (ss) kets, prefixes of at least krow_min_kets kets (256 unless
+    // set), when all NP + NQ rows fit): the bra pair's density rows at the same columns as its K rows, staged in LDS
+    // after them, so that the K digestion reads P(bra function, ket function) from LDS,
+    // not global memory ((ps|ss) at 4160 functions: 324 -> 239 ms).  Compiled for the
+    // (..|ss) class pairs only: with p or d kets the rows rarely fit twice, and the second
+    // path alone slowed those kernels by 5-13 %.
+    constexpr bool PR = KROWS && LR == 0 && LS == 0;
+    const int npr = PR && b.jend >= a.krow_min_kets ? a.prow_rows : 0;  // 0 or NP + NQ, uniform
+    double *const prow = krow + nkr * W;
     if (KROWS) {
         for (int t = threadIdx.x; t < nkr * W; t += blockDim.x) krow[t] = 0.0;
+        for (int t = threadIdx.x; PR && t < npr * W; t += blockDim.x) {
+            const int row = t / W, cc = t - row * W;
+            const int col = ldc(a.kcol_map + (cc < nR ? a.kcol_lofs[LR] + cc : a.kcol_lofs[LS] + cc - nR));
+            prow[t] = b.P[(size_t)(row < NP ? fp + row : fq + row - NP) * ld + col];
+        }
         __syncthreads();
     }
     // K(row, col) += v: row = bra function fp + p (p < NP) or fq + q (NP + q); col the
@@ -643,19 +660,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
                 ccs = (LR == LS ? 0 : nR) + lo16(kc);
             }
             double Psq[NS * NQ], Prq[NR * NQ], Psp[NS * NP], Prp[NR * NP];
+            if (PR && npr) {  // uniform: every row staged
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+                for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fq + q) * ld + fs + s];
+                    for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = prow[(NP + q) * W + ccs + s];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fq + q) * ld + fr + r];
-            }
+                    for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = prow[(NP + q) * W + ccr + r];
+                }
 #pragma unroll
-            for (int p = 0; p < NP; ++p) {
+                for (int p = 0; p < NP; ++p) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fp + p) * ld + fs + s];
+                    for (int s = 0; s < NS; ++s) Psp[s * NP + p] = prow[p * W + ccs + s];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fp + p) * ld + fr + r];
+                    for (int r = 0; r < NR; ++r) Prp[r * NP + p] = prow[p * W + ccr + r];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) Psq[s * NQ + q] = P[(size_t)(fq + q) * ld + fs + s];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) Prq[r * NQ + q] = P[(size_t)(fq + q) * ld + fr + r];
+                }
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) Psp[s * NP + p] = P[(size_t)(fp + p) * ld + fs + s];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) Prp[r * NP + p] = P[(size_t)(fp + p) * ld + fr + r];
+                }
             }
 #pragma unroll
             for (int p = 0; p < NP; ++p)

@@ -477,6 +477,7 @@ struct tgpu_ctx {
     // TGPU_KROW_BUDGET sets it (tests use it to exercise partial rows on small systems)
     int krow_budget = 0;
     int krow_min_kets = 256;  // TGPU_KROW_MIN_KETS
+    int prow_enable = 1;      // density rows in LDS in the KROWS form (TGPU_PROW=0: off)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
     bool timed = false;
     bool profiling = false;
@@ -745,6 +746,7 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
     c->n_mol = n_mol;
     if (const char *kb = std::getenv("TGPU_KROW_BUDGET")) c->krow_budget = std::atoi(kb);
     if (const char *km = std::getenv("TGPU_KROW_MIN_KETS")) c->krow_min_kets = std::atoi(km);
+    if (const char *pr = std::getenv("TGPU_PROW")) c->prow_enable = std::atoi(pr) != 0;
     c->atom_off.assign(n_mol + 1, 0);
     c->shell_off.assign(n_mol + 1, 0);
     c->pair_off.assign(n_mol + 1, 0);
@@ -1203,6 +1205,8 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     a.krow_rows = 0;
     a.krow_budget = c->krow_budget;
     a.krow_min_kets = c->krow_min_kets;
+    a.prow_rows = 0;
+    a.prow_enable = c->prow_enable;
     a.kcol = c->n_mol == 1 ? c->d_kcol.p : nullptr;
     a.kcol_map = c->n_mol == 1 ? c->d_kcol_map.p : nullptr;
     for (int l = 0; l <= TGPU_LMAX + 1; ++l) a.kcol_lofs[l] = c->kcol_lofs[l];
