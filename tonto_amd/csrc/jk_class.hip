@@ -60,7 +60,17 @@ int TGPU_LAUNCH_NAME_X(TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS)(const JKArgs &a, int 
             b.prow_rows = TGPU_LR == 0 && TGPU_LS == 0 && a.prow_enable && nkr == NP + NQ && pfit >= NP + NQ
                               ? NP + NQ
                               : 0;
-            const size_t rows = (size_t)(nkr + b.prow_rows) * row_bytes;
+            size_t rows = (size_t)(nkr + b.prow_rows) * row_bytes;
+            // the bra pair's two P_max rows, in what is left, unless they lower the number of
+            // work-groups a CU holds ((pp|ss) at 4160 functions: 2 -> 1, 99 -> 130 ms; (ds|ps),
+            // at one either way: 259 -> 239 ms)
+            const size_t pm_bytes = 2 * (size_t)a.n_shell * sizeof(double);
+            constexpr size_t kCuLds = 160 * 1024;
+            b.pmrow = a.pmrow && a.n_shell > 0 && budget >= kStatic + rows + pm_bytes &&
+                              kCuLds / (kStatic + rows + pm_bytes) == kCuLds / (kStatic + rows)
+                          ? 1
+                          : 0;
+            if (b.pmrow) rows += pm_bytes;
             if (mixed) jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, true, true><<<a.n_bra, 256, rows, s>>>(b);
             else jk_kernel<TGPU_LP, TGPU_LQ, TGPU_LR, TGPU_LS, false, true><<<a.n_bra, 256, rows, s>>>(b);
             return 0;

@@ -339,6 +339,9 @@ struct JKArgs {
     // KROWS form: density rows staged in LDS next to the K rows (set per launch;
     // prow_enable 0 keeps every density read global)
     int prow_rows, prow_enable;
+    // KROWS form, single molecules: n_shell shells; pmrow (set per launch) 1 when the bra
+    // pair's two P_max rows (2 n_shell doubles) are staged in LDS after the density rows
+    int n_shell, pmrow;
     // KROWS form, single molecules: the K columns a class pair touches are the functions
     // of its ket shells' two angular momenta, so an LDS row holds only those, numbered
     // within each momentum.  kcol[pair] = pack16(number of the pair's first function
@@ -438,7 +441,10 @@ struct KetTest {
 // Tonto's remaining per-quartet tests (molecule.fock.foo:1412, 1432-1472 and
 // Schwarz_test 3497-3536), evaluated in Tonto's operand order.  Returns false if the
 // quartet contributes to neither J nor K.
-__device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int j, KetTest &t)
+// pmr (optional, LDS): P_max(bra shell p, x) for the molecule's nsh shells x, then
+// P_max(bra shell q, x): the same values as the Tonto-pair lookups, read from LDS.
+__device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int j, KetTest &t,
+                                         const double *pmr = nullptr, int nsh = 0)
 {
     t.sk = b.kb + j;
     const int4 r0 = ldc(a.pr.rec0 + t.sk);
@@ -450,8 +456,9 @@ __device__ __forceinline__ bool test_ket(const JKArgs &a, const BraInfo &b, int 
     const int kr = hi16(r0.y), ks = lo16(r0.y);
     const double Pmax_k = ldc(b.Pmax + tk);
     const double JP = bhi ? fmax(b.Pmax_b, Pmax_k) : fmax(Pmax_k, b.Pmax_b);
-    const double KP = fmax(fmax(ldc(b.Pmax + tonto_pair(b.bp, kr)), ldc(b.Pmax + tonto_pair(b.bp, ks))),
-                           fmax(ldc(b.Pmax + tonto_pair(b.bq, kr)), ldc(b.Pmax + tonto_pair(b.bq, ks))));
+    const double KP = pmr ? fmax(fmax(pmr[kr], pmr[ks]), fmax(pmr[nsh + kr], pmr[nsh + ks]))
+                          : fmax(fmax(ldc(b.Pmax + tonto_pair(b.bp, kr)), ldc(b.Pmax + tonto_pair(b.bp, ks))),
+                                 fmax(ldc(b.Pmax + tonto_pair(b.bq, kr)), ldc(b.Pmax + tonto_pair(b.bq, ks))));
     if (a.sph_screen) {
         // Schwarz_inequality_test (molecule.fock.foo:3454-3495): the largest of the six
         // density blocks, skip below TOL(20) or unless Pmax*I_max(ab)*I_max(cd) > cutoff
@@ -555,7 +562,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     constexpr bool PR = KROWS && LR == 0 && LS == 0;
     const int npr = PR && b.jend >= a.krow_min_kets ? a.prow_rows : 0;  // 0 or NP + NQ, uniform
     double *const prow = krow + nkr * W;
+    // P_max rows (KROWS, prefixes of at least half the shell count): the screening's four
+    // cross-pair density maxima of each ket come from LDS
+    const int nsh = a.n_shell;
+    double *const pmr = KROWS && a.pmrow && 2 * b.jend >= nsh ? prow + a.prow_rows * W : nullptr;
     if (KROWS) {
+        if (pmr)
+            for (int x = threadIdx.x; x < nsh; x += blockDim.x) {
+                pmr[x] = ldc(b.Pmax + tonto_pair(b.bp, x));
+                pmr[nsh + x] = ldc(b.Pmax + tonto_pair(b.bq, x));
+            }
         for (int t = threadIdx.x; t < nkr * W; t += blockDim.x) krow[t] = 0.0;
         for (int t = threadIdx.x; PR && t < npr * W; t += blockDim.x) {
             const int row = t / W, cc = t - row * W;
@@ -754,7 +770,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
         if (j0 < b.jend) {
             const int j = j0 + lane;
             KetTest t;
-            const bool ok = j < b.jend && test_ket(a, b, j, t);
+            const bool ok = j < b.jend && test_ket(a, b, j, t, pmr, nsh);
             const unsigned long long m = __ballot(ok);
             if (ok) {
                 const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
@@ -782,7 +798,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
     for (int j0 = KROWS ? 64 * wave : 0; j0 < b.jend; j0 += KROWS ? 256 : 64) {
         const int j = j0 + lane;
         KetTest t;
-        if (j < b.jend && test_ket(a, b, j, t))
+        if (j < b.jend && test_ket(a, b, j, t, pmr, nsh))
             quartet(j, (int)t.skipJ | ((int)t.skipK << 1) | ((int)t.skip_lists << 2) | (t.halvings << 3),
                     t.fr, t.fs, make_view(a.pr, a.pm, b.sb, t.skip_lists));
     }

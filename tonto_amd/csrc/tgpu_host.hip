@@ -478,6 +478,7 @@ struct tgpu_ctx {
     int krow_budget = 0;
     int krow_min_kets = 256;  // TGPU_KROW_MIN_KETS
     int prow_enable = 1;      // density rows in LDS in the KROWS form (TGPU_PROW=0: off)
+    int pmrow_enable = 1;     // P_max rows in LDS in the KROWS form (TGPU_PMROW=0: off)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
     bool timed = false;
     bool profiling = false;
@@ -747,6 +748,7 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
     if (const char *kb = std::getenv("TGPU_KROW_BUDGET")) c->krow_budget = std::atoi(kb);
     if (const char *km = std::getenv("TGPU_KROW_MIN_KETS")) c->krow_min_kets = std::atoi(km);
     if (const char *pr = std::getenv("TGPU_PROW")) c->prow_enable = std::atoi(pr) != 0;
+    if (const char *pm = std::getenv("TGPU_PMROW")) c->pmrow_enable = std::atoi(pm) != 0;
     c->atom_off.assign(n_mol + 1, 0);
     c->shell_off.assign(n_mol + 1, 0);
     c->pair_off.assign(n_mol + 1, 0);
@@ -1207,6 +1209,8 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     a.krow_min_kets = c->krow_min_kets;
     a.prow_rows = 0;
     a.prow_enable = c->prow_enable;
+    a.n_shell = c->n_mol == 1 ? (int)c->sh_l.size() : 0;
+    a.pmrow = c->pmrow_enable;
     a.kcol = c->n_mol == 1 ? c->d_kcol.p : nullptr;
     a.kcol_map = c->n_mol == 1 ? c->d_kcol_map.p : nullptr;
     for (int l = 0; l <= TGPU_LMAX + 1; ++l) a.kcol_lofs[l] = c->kcol_lofs[l];
