@@ -43,12 +43,16 @@ METRIC = "SCF wall-time & ERI quartets/s, mixed vs FP64 GPU at |ΔE|≤1e-8 Eh"
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and FP64 matrix), AMD published
 N_WATER, BASIS, SEED = 64, "cc-pVTZ", 0
 SCHWARZ, FP32_THR = 1e-11, 1e-6
-# HBM bytes per build of the dominant kernel on this workload (sum over the class pair's
-# launches of one build, 16 dispatches), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes
-# scaled by what the same counters read for a 256 MiB copy (x2.000 and x1.000): the
-# profiles/ traffic summary of (H2O)64 / cc-pVTZ
-TRAFFIC = {"jk class (10|00) mixed": {"bytes": 1.680240e+12, "read": 1.613008e+12, "written": 6.723149e+10,
-                                     "dispatches": 16}}
+# HBM bytes per build of the class pairs that can be the dominant kernel on this workload
+# (sum over the class pair's launches of one mixed build), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by what the same counters read for a 256 MiB copy (x2.000 and
+# x1.000): the profiles/ traffic summary of (H2O)64 / cc-pVTZ
+TRAFFIC = {
+    "jk class (21|21) mixed": {"bytes": 2.512824e+11, "read": 1.807234e+11, "written": 7.055901e+10, "dispatches": 6},
+    "jk class (20|10) mixed": {"bytes": 9.951025e+11, "read": 8.881830e+11, "written": 1.069195e+11, "dispatches": 12},
+    "jk class (10|00) mixed": {"bytes": 5.139057e+11, "read": 4.466742e+11, "written": 6.723149e+10, "dispatches": 16},
+    "jk class (10|10) mixed": {"bytes": 1.251133e+12, "read": 1.139521e+12, "written": 1.116115e+11, "dispatches": 10},
+}
 
 
 def log(*a):
@@ -368,9 +372,9 @@ def main():
         roof["traffic_read_written"] = [tr["read"], tr["written"]]
         roof["traffic_per_dispatch"] = tr["bytes"] / tr["dispatches"]
         roof["traffic_note"] = ("HBM-side bytes per build from the rocprofv3 FETCH_SIZE/WRITE_SIZE "
-                                "profile of this workload; the reads are the kets' primitive-pair "
-                                "records (64 B per primitive pair per quartet), served mostly by the "
-                                "Infinity Cache; algorithmic bytes per build are a few MB")
+                                "profile of this workload (memory-side requests, mostly served by the "
+                                "Infinity Cache): the kets' primitive-pair records and pair data, the "
+                                "density and the J/K atomics; algorithmic bytes per build are a few MB")
     cfg = {"workload": f"(H2O){a.waters} RHF/{a.basis} Cartesian, {mol.n_bf} basis functions, "
                        f"seeded cubic-lattice cluster, Schwarz {a.schwarz:g}; one step = one J/K build at the "
                        f"converged density, FP32 below Schwarz bound {FP32_THR:g}",
