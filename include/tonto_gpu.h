@@ -144,10 +144,11 @@ int tgpu_make_r_JK_device(tgpu_ctx *ctx, const double *P, double *J, double *K, 
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same results and flags (TGPU_JK_MIXED,
  * TGPU_J_ONLY, TGPU_K_ONLY; not TGPU_JK_GRAPH); every device buffer the build writes
- * besides J and K (the X/Y accumulators, shell-pair density maxima, counters and the
- * f-shell workspace) is taken from one library-owned array of 64 MiB per GPU, which each
- * GPU holds until the process ends.  Limit: 2 n_bf^2 doubles plus the shell-pair maxima
- * and one workspace slot must fit 64 MiB (about n_bf <= 2000); larger systems get
+ * besides J and K (the X/Y accumulators, the pair-ordered density and ket-side J blocks,
+ * shell-pair density maxima, counters and the f-shell workspace) is taken from one
+ * library-owned array of 64 MiB per GPU, which each GPU holds until the process ends.
+ * Limit: about 3 n_bf^2 doubles plus the shell-pair maxima and one workspace slot must
+ * fit 64 MiB (about n_bf <= 1650); larger systems get
  * TGPU_E_UNSUPPORTED.  tgpu_schwarz must have run (TGPU_E_STATE otherwise) and the
  * context's device must be current.  All calls on one GPU share the array: calls that
  * overlap on one GPU (two streams, two threads) race.  Asynchronous on the context's

@@ -339,6 +339,12 @@ struct JKArgs {
     // KROWS form: density rows staged in LDS next to the K rows (set per launch;
     // prow_enable 0 keeps every density read global)
     int prow_rows, prow_enable;
+    // Pair-ordered ket blocks (lane form): PP holds each pair's density block, XP takes
+    // each pair's ket-side J (added into X after the class launches); a ket pair sk of
+    // the launch's ket bin has its block at kblk_off + sk * NR * NS
+    const double *PP;
+    double *XP;
+    long long kblk_off;
     // KROWS form, single molecules: n_shell shells; pmrow (set per launch) 1 when the bra
     // pair's two P_max rows (2 n_shell doubles) are staged in LDS after the density rows
     int n_shell, pmrow;
@@ -638,18 +644,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_wave
             j_bra = b.tb < tk;
         }
         if (!t.skipJ) {
+            // the ket's density block and its ket-side J through the pair-ordered blocks:
+            // neighbouring lanes hold neighbouring kets, so these loads and atomics fall on
+            // few cache lines instead of one line each in P and X
+            const long long kb0 = a.kblk_off + (long long)t.sk * NRS;
             double Prs[NRS];
 #pragma unroll
-            for (int r = 0; r < NR; ++r)
-#pragma unroll
-                for (int s = 0; s < NS; ++s) Prs[r * NS + s] = P[(size_t)(fr + r) * ld + fs + s];
+            for (int rs = 0; rs < NRS; ++rs) Prs[rs] = a.PP[kb0 + rs];
             if (j_ket) {
 #pragma unroll
                 for (int rs = 0; rs < NRS; ++rs) {
                     double acc = 0.0;
 #pragma unroll
                     for (int pq = 0; pq < NPQ; ++pq) acc += I[pq * NRS + rs] * Ppq[pq];
-                    unsafeAtomicAdd(&b.X[(size_t)(fr + rs / NS) * ld + fs + rs % NS], f * acc);
+                    unsafeAtomicAdd(&a.XP[kb0 + rs], f * acc);
                 }
             }
             if (j_bra) {

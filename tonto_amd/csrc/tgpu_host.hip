@@ -160,6 +160,59 @@ __global__ void post_kernel(const double *__restrict__ X, const double *__restri
     }
 }
 
+// Pair-ordered blocks: pair s (device order) of bin k, class (la, lb), owns the
+// ncart(la) x ncart(lb) doubles at bin_boff[k] + (s - bin_start[k]) ncart(la) ncart(lb)
+// of PP and XP; its block of a molecule matrix has rows fr.., columns fs.. (rec0.w).
+struct PairBlocks {
+    const int *bin_start;      // [NBIN + 1]
+    const long long *bin_boff;  // [NBIN + 1]
+    const int4 *rec0, *rec1;
+    const long long *mat_off;
+    const int *nbf;
+    long long n_pairs;
+};
+
+__device__ __forceinline__ void pair_block(const PairBlocks &pb, int s, long long &off, int &nr, int &ns)
+{
+    int k = 0;  // the last bin starting at or before s holds it
+    while (k + 1 < NBIN && pb.bin_start[k + 1] <= s) ++k;
+    const int cls = k / NBK;
+    int lp = 0;
+    while ((lp + 1) * (lp + 2) / 2 <= cls) ++lp;
+    nr = ncart(lp);
+    ns = ncart(cls - lp * (lp + 1) / 2);
+    off = pb.bin_boff[k] + (long long)(s - pb.bin_start[k]) * nr * ns;
+}
+
+// PP <- each pair's block of P (one thread per pair)
+__global__ void pair_gather_kernel(const double *__restrict__ P, double *__restrict__ PP, PairBlocks pb)
+{
+    const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (s >= pb.n_pairs) return;
+    long long off;
+    int nr, ns;
+    pair_block(pb, (int)s, off, nr, ns);
+    const int w = pb.rec0[s].w, fr = (int)((unsigned)w >> 16), fs = w & 0xffff, m = pb.rec1[s].w;
+    const int ld = pb.nbf[m];
+    const double *Pm = P + pb.mat_off[m];
+    for (int u = 0; u < nr * ns; ++u) PP[off + u] = Pm[(size_t)(fr + u / ns) * ld + fs + u % ns];
+}
+
+// X += each pair's ket-side J block from XP (one thread per pair; the blocks of different
+// pairs are different elements of X)
+__global__ void pair_scatter_kernel(const double *__restrict__ XP, double *__restrict__ X, PairBlocks pb)
+{
+    const long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (s >= pb.n_pairs) return;
+    long long off;
+    int nr, ns;
+    pair_block(pb, (int)s, off, nr, ns);
+    const int w = pb.rec0[s].w, fr = (int)((unsigned)w >> 16), fs = w & 0xffff, m = pb.rec1[s].w;
+    const int ld = pb.nbf[m];
+    double *Xm = X + pb.mat_off[m];
+    for (int u = 0; u < nr * ns; ++u) Xm[(size_t)(fr + u / ns) * ld + fs + u % ns] += XP[off + u];
+}
+
 // Accumulator copies (X and Y each hold `copies` copies of n elements): the sum into the
 // first copy, before post_kernel.
 __global__ void acc_reduce_kernel(double *__restrict__ X, double *__restrict__ Y, size_t n, int copies)
@@ -468,6 +521,12 @@ struct tgpu_ctx {
     DBuf<PrimRec> d_prim;
     DBuf<double> d_X, d_Y, d_Pmax, d_scal, d_hP, d_hJ, d_hK;
     int acc_copies = 1;  // accumulator copies in d_X, d_Y (1 or TGPU_ACC_COPIES)
+    long long n_pairs_dev() const { return bin_start[NBIN]; }  // pairs in device order
+    // pair-ordered ket blocks: offsets per bin, their total, the device copies
+    long long bin_boff[NBIN + 1] = {0};
+    DBuf<int> d_bin_start;
+    DBuf<long long> d_bin_boff;
+    DBuf<double> d_PP, d_XP;
     DBuf<double> d_hP2, d_hJ2, d_hK2, d_uJ, d_uK;  // UHF: host-entry staging, ket-side pass
     DBuf<unsigned long long> d_counters;
     // wavefront-form workspace (classes whose transfer buffers exceed the LDS budget)
@@ -838,6 +897,13 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
         }
     }
     c->bin_start[NBIN] = (int)c->tonto.size();
+    c->bin_boff[0] = 0;
+    for (int k = 0; k < NBIN; ++k) {
+        int lp = 0;
+        while ((lp + 1) * (lp + 2) / 2 <= k / NBK) ++lp;
+        const int lq = k / NBK - lp * (lp + 1) / 2;
+        c->bin_boff[k + 1] = c->bin_boff[k] + (long long)(c->bin_start[k + 1] - c->bin_start[k]) * ncart(lp) * ncart(lq);
+    }
     c->bin_max_seg.assign(NBIN, 0);
     for (int m = 0; m < n_mol; ++m)
         for (int k = 0; k < NBIN; ++k)
@@ -893,6 +959,14 @@ extern "C" int tgpu_create_batch(const tgpu_system *sys, int n_mol, tgpu_ctx **o
                         : 1;
     if (const char *ac = std::getenv("TGPU_ACC_COPIES"))
         if (std::atoi(ac) == 1) c->acc_copies = 1;
+    {
+        const std::vector<int> bs(c->bin_start, c->bin_start + NBIN + 1);
+        const std::vector<long long> bo(c->bin_boff, c->bin_boff + NBIN + 1);
+        if ((rc = dupload(c, c->d_bin_start, bs)) || (rc = dupload(c, c->d_bin_boff, bo)) ||
+            (rc = dalloc(c, c->d_PP, (size_t)std::max(1LL, c->bin_boff[NBIN]))) ||
+            (rc = dalloc(c, c->d_XP, (size_t)std::max(1LL, c->bin_boff[NBIN]))))
+            return rc;
+    }
     if ((rc = dalloc(c, c->d_X, (size_t)c->n_elem * c->acc_copies)) ||
         (rc = dalloc(c, c->d_Y, (size_t)c->n_elem * c->acc_copies)) ||
         (rc = dalloc(c, c->d_Pmax, (size_t)c->n_pairs)) || (rc = dalloc(c, c->d_scal, 2 * (size_t)n_mol)) ||
@@ -921,6 +995,7 @@ extern "C" int tgpu_destroy(tgpu_ctx *c)
     dfree(c->d_kcol); dfree(c->d_kcol_map);
     dfree(c->d_A); dfree(c->d_AB); dfree(c->d_maxI); dfree(c->d_maxI_mol); dfree(c->d_prim);
     dfree(c->d_X); dfree(c->d_Y);
+    dfree(c->d_bin_start); dfree(c->d_bin_boff); dfree(c->d_PP); dfree(c->d_XP);
     dfree(c->d_Pmax); dfree(c->d_scal); dfree(c->d_hP); dfree(c->d_hJ); dfree(c->d_hK);
     dfree(c->d_hP2); dfree(c->d_hJ2); dfree(c->d_hK2); dfree(c->d_uJ); dfree(c->d_uK);
     dfree(c->d_counters);
@@ -1127,12 +1202,13 @@ struct BuildBufs {
     double *ws;
     long long ws_doubles;
     int acc_copies;  // copies of X and Y (n_elem doubles apart)
+    double *PP, *XP;  // pair-ordered ket blocks (bin_boff[NBIN] doubles each)
 };
 
 static BuildBufs ctx_bufs(const tgpu_ctx *c)
 {
-    return BuildBufs{c->d_X.p,       c->d_Y.p,    c->d_Pmax.p,      c->d_scal.p,
-                     c->d_counters.p, c->d_ws.p, c->ws_doubles, c->acc_copies};
+    return BuildBufs{c->d_X.p,       c->d_Y.p,    c->d_Pmax.p,      c->d_scal.p,  c->d_counters.p,
+                     c->d_ws.p,      c->ws_doubles, c->acc_copies, c->d_PP.p, c->d_XP.p};
 }
 
 // Largest per-slot workspace (doubles) of the wavefront-form class pairs this basis has
@@ -1176,6 +1252,11 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     fill_zero_kernel<<<grid_for(n_acc), 256, 0, s>>>(B.X, n_acc);
     fill_zero_kernel<<<grid_for(n_acc), 256, 0, s>>>(B.Y, n_acc);
     fill_zero_u64_kernel<<<(3 * NQCLS + 255) / 256, 256, 0, s>>>(B.counters, 3 * NQCLS);
+    // the pair-ordered ket blocks: each pair's density block in, the ket-side J out
+    const PairBlocks pbk{c->d_bin_start.p, c->d_bin_boff.p, reinterpret_cast<const int4 *>(c->d_rec0.p),
+                         reinterpret_cast<const int4 *>(c->d_rec1.p), c->d_mat_off.p, c->d_nbf.p, c->n_pairs_dev()};
+    pair_gather_kernel<<<(int)((pbk.n_pairs + 255) / 256), 256, 0, s>>>(P, B.PP, pbk);
+    fill_zero_kernel<<<grid_for((size_t)c->bin_boff[NBIN]), 256, 0, s>>>(B.XP, (size_t)c->bin_boff[NBIN]);
     pmax_kernel<<<grid_for((size_t)c->n_pairs), 256, 0, s>>>(c->spherical ? Psph : P, P2, ba, B.Pmax);
     mol_max_kernel<<<c->n_mol, 256, 0, s>>>(B.Pmax, ba, c->d_maxI_mol.p, B.scal);
     if (rec_start) (void)hipEventRecord(c->ev0, s);
@@ -1199,6 +1280,8 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     a.fp32_thr = c->fp32_thr;
     a.X = B.X;
     a.Y = B.Y;
+    a.PP = B.PP;
+    a.XP = B.XP;
     a.acc_stride = B.acc_copies > 1 ? c->n_elem : 0;
     a.counters = B.counters;
     a.ws = B.ws;
@@ -1233,6 +1316,12 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
                 a.n_bra = c->bin_start[X + 1] - c->bin_start[X];
                 a.same_bin = X == Y;
                 a.n_ket_max = c->bin_max_seg[Y];
+                {
+                    int lp = 0;
+                    while ((lp + 1) * (lp + 2) / 2 <= CY) ++lp;
+                    const int nrs = ncart(lp) * ncart(CY - lp * (lp + 1) / 2);
+                    a.kblk_off = c->bin_boff[Y] - (long long)c->bin_start[Y] * nrs;
+                }
                 // the LDS-row form is for single large molecules (0 keeps batches of many,
                 // mostly small molecules at four bra pairs per work-group)
                 a.n_ket_avg = c->n_mol == 1 ? c->bin_start[Y + 1] - c->bin_start[Y] : 0;
@@ -1260,6 +1349,7 @@ static int enqueue_build(tgpu_ctx *c, const BuildBufs &B, const double *P, doubl
     if (rec_end) (void)hipEventRecord(c->ev1, s);
     if (B.acc_copies > 1)
         acc_reduce_kernel<<<grid_for(c->n_elem), 256, 0, s>>>(B.X, B.Y, (size_t)c->n_elem, B.acc_copies);
+    pair_scatter_kernel<<<(int)((pbk.n_pairs + 255) / 256), 256, 0, s>>>(B.XP, B.X, pbk);
     post_kernel<<<dim3(grid_for(c->n_elem / c->n_mol + 1), c->n_mol), 256, 0, s>>>(B.X, B.Y, ba, J, K);
     if (c->spherical) {
         const SphMaps sm = sph_maps(c);
@@ -1357,12 +1447,14 @@ extern "C" int tgpu_make_r_JK_device_noalloc(tgpu_ctx *c, const double *P, doubl
     const size_t b_pmax = up((size_t)c->n_pairs * sizeof(double));
     const size_t b_scal = up(2 * (size_t)c->n_mol * sizeof(double));
     const size_t b_cnt = up(3 * (size_t)NQCLS * sizeof(unsigned long long));
-    const size_t fixed = 2 * b_mat + b_pmax + b_scal + b_cnt;
+    const size_t b_blk = up((size_t)std::max(1LL, c->bin_boff[NBIN]) * sizeof(double));
+    const size_t fixed = 2 * b_mat + b_pmax + b_scal + b_cnt + 2 * b_blk;
     const size_t ws_slot = workspace_need(c) * sizeof(double);  // one slot of the largest class
     if (fixed + ws_slot > TGPU_NOALLOC_SCRATCH_BYTES) {
         c->err = "tgpu_make_r_JK_device_noalloc: this system needs " + std::to_string(fixed + ws_slot) +
                  " bytes of scratch (2 n_bf^2 doubles for " + std::to_string(c->n_elem) +
-                 " matrix elements, the shell-pair maxima and a workspace slot); the maximum is " +
+                 " matrix elements, two pair-ordered blocks of about n_bf^2/2, the shell-pair maxima "
+                 "and a workspace slot); the maximum is " +
                  std::to_string(TGPU_NOALLOC_SCRATCH_BYTES) + " bytes";
         return TGPU_E_UNSUPPORTED;
     }
@@ -1388,6 +1480,8 @@ extern "C" int tgpu_make_r_JK_device_noalloc(tgpu_ctx *c, const double *P, doubl
     B.Pmax = reinterpret_cast<double *>(base + off); off += b_pmax;
     B.scal = reinterpret_cast<double *>(base + off); off += b_scal;
     B.counters = reinterpret_cast<unsigned long long *>(base + off); off += b_cnt;
+    B.PP = reinterpret_cast<double *>(base + off); off += b_blk;
+    B.XP = reinterpret_cast<double *>(base + off); off += b_blk;
     B.ws = ws_slot ? reinterpret_cast<double *>(base + off) : nullptr;
     B.ws_doubles = ws_slot ? (long long)((TGPU_NOALLOC_SCRATCH_BYTES - off) / sizeof(double)) : 0;
     B.acc_copies = 1;
