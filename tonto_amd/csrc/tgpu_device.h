@@ -1147,7 +1147,7 @@ __device__ __forceinline__ void wave_digest(const JKArgs &a, const BraInfo &b, c
                 double acc = 0.0;
 #pragma unroll 4
                 for (int pq = 0; pq < NPQ; ++pq) acc += (double)I[pq * NRS + u] * L.Ppq[pq];
-                unsafeAtomicAdd(&b.X[(size_t)(fr + u / NS) * ld + fs + u % NS], f * acc);
+                unsafeAtomicAdd(&a.XP[a.kblk_off + (long long)t.sk * NRS + u], f * acc);
             }
         }
         if (j_bra) {
@@ -1271,7 +1271,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> *Lt, d
         if (live) {
             if constexpr (NRS <= S && NS * NQ <= S && NR * NQ <= S && NS * NP <= S && NR * NP <= S) {
                 const int u = tl;
-                const double v0 = u < NRS ? P[(size_t)(fr + u / NS) * ld + fs + u % NS] : 0.0;
+                const double v0 = u < NRS ? a.PP[a.kblk_off + (long long)t.sk * NRS + u] : 0.0;
                 const double v1 = u < NS * NQ ? P[(size_t)(fq + u % NQ) * ld + fs + u / NQ] : 0.0;
                 const double v2 = u < NR * NQ ? P[(size_t)(fq + u % NQ) * ld + fr + u / NQ] : 0.0;
                 const double v3 = u < NS * NP ? P[(size_t)(fp + u % NP) * ld + fs + u / NP] : 0.0;
@@ -1282,7 +1282,7 @@ __device__ void wave_bra(const JKArgs &a, int bi, WaveLDS<LP, LQ, LR, LS> *Lt, d
                 if (u < NS * NP) L.Psp[u] = v3;
                 if (u < NR * NP) L.Prp[u] = v4;
             } else {
-                for (int u = tl; u < NRS; u += S) L.Prs[u] = P[(size_t)(fr + u / NS) * ld + fs + u % NS];
+                for (int u = tl; u < NRS; u += S) L.Prs[u] = a.PP[a.kblk_off + (long long)t.sk * NRS + u];
                 for (int u = tl; u < NS * NQ; u += S) L.Psq[u] = P[(size_t)(fq + u % NQ) * ld + fs + u / NQ];
                 for (int u = tl; u < NR * NQ; u += S) L.Prq[u] = P[(size_t)(fq + u % NQ) * ld + fr + u / NQ];
                 for (int u = tl; u < NS * NP; u += S) L.Psp[u] = P[(size_t)(fp + u % NP) * ld + fs + u / NP];
