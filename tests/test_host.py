@@ -9,7 +9,7 @@ from tonto_amd.scf import run_rhf
 
 
 def test_component_order_matches_gaussian_data():
-    # foofiles/gaussian_data.foo:295-479 (d and f orders quoted in the SURVEY)
+    # foofiles/gaussian_data.foo:295-479: Tonto's d and f component orders
     assert basis.cart_components(2) == [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]
     assert basis.cart_components(3) == [(3, 0, 0), (0, 3, 0), (0, 0, 3), (2, 1, 0), (2, 0, 1),
                                         (1, 2, 0), (0, 2, 1), (1, 0, 2), (0, 1, 2), (1, 1, 1)]
