@@ -86,3 +86,22 @@ def test_sixteen_bit_packing_round_trip(tmp_path):
                            "-I", os.path.join(root, "tonto_amd", "csrc"), str(src), "-o", str(exe)])
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip() == "0"
+
+
+def test_bench_traffic_table_names_class_pairs():
+    """bench.py attaches rocprofv3 traffic to the roofline of the dominant class pair by
+    the name it gives that kernel ("jk class (ab|cd) mixed"); every entry of its table must
+    name a class pair of the library and carry consistent byte counts."""
+    import importlib.util
+    import os
+    from tonto_amd.flops import class_pairs
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py")
+    spec = importlib.util.spec_from_file_location("bench_module", path)
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    names = {"jk class ({}{}|{}{}) mixed".format(*cp) for cp in class_pairs()}
+    assert bench.TRAFFIC
+    for name, t in bench.TRAFFIC.items():
+        assert name in names, name
+        assert abs(t["bytes"] - t["read"] - t["written"]) <= 1e-6 * t["bytes"]
+        assert t["dispatches"] > 0 and t["read"] > 0 and t["written"] > 0
